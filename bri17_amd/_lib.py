@@ -1,0 +1,83 @@
+"""ctypes binding of libbri17_amd.so (the C ABI declared in include/bri17_amd.h).
+
+The library is the only compute path: if it is missing this module raises
+instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbri17_amd.so")
+
+BRI17_OK = 0
+BRI17_ERR_INVALID = 1
+BRI17_ERR_UNSUPPORTED = 2
+BRI17_ERR_HIP = 3
+BRI17_ERR_NOMEM = 4
+BRI17_ERR_STATE = 5
+BRI17_F32 = 0
+BRI17_F64 = 1
+
+# name -> (restype, argtypes); mirrors include/bri17_amd.h
+_c = ctypes
+_ip = _c.POINTER(_c.c_int)
+_dp = _c.POINTER(_c.c_double)
+_fp = _c.POINTER(_c.c_float)
+_vp = _c.c_void_p
+SIGNATURES = {
+    "bri17_last_error": (_c.c_char_p, []),
+    "bri17_version": (_c.c_char_p, []),
+    "bri17_grid_node_at": (_c.c_int, [_c.c_int, _ip, _ip, _ip]),
+    "bri17_grid_cell_nodes": (_c.c_int, [_c.c_int, _ip, _c.c_int, _ip]),
+    "bri17_modal_strain_displacement_f64": (_c.c_int, [_c.c_int, _ip, _dp, _ip, _dp]),
+    "bri17_modal_strain_displacement_f32": (_c.c_int, [_c.c_int, _ip, _fp, _ip, _fp]),
+    "bri17_modal_stiffness_f64": (_c.c_int, [_c.c_int, _ip, _dp, _c.c_double, _c.c_double, _ip, _dp]),
+    "bri17_modal_stiffness_f32": (_c.c_int, [_c.c_int, _ip, _fp, _c.c_float, _c.c_float, _ip, _fp]),
+    "bri17_modal_eigenstress_to_opposite_strain_f64": (
+        _c.c_int, [_c.c_int, _ip, _dp, _c.c_double, _c.c_double, _ip, _dp, _dp]),
+    "bri17_modal_eigenstress_to_opposite_strain_f32": (
+        _c.c_int, [_c.c_int, _ip, _fp, _c.c_float, _c.c_float, _ip, _fp, _fp]),
+    "bri17_operator_create": (
+        _c.c_int, [_c.c_int, _ip, _dp, _c.c_double, _c.c_double, _c.c_int, _c.POINTER(_vp)]),
+    "bri17_operator_destroy": (_c.c_int, [_vp]),
+    "bri17_operator_set_material": (_c.c_int, [_vp, _c.c_double, _c.c_double]),
+    "bri17_operator_info": (_c.c_int, [_vp, _c.POINTER(_c.c_int64), _c.POINTER(_c.c_size_t)]),
+    "bri17_operator_apply": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "bri17_operator_solve": (_c.c_int, [_vp, _vp, _vp, _vp]),
+    "bri17_operator_apply_host": (_c.c_int, [_vp, _vp, _vp]),
+    "bri17_operator_solve_host": (_c.c_int, [_vp, _vp, _vp]),
+    "bri17_operator_profile": (_c.c_int, [_vp, _c.c_int]),
+    "bri17_operator_profile_read": (_c.c_int, [_vp, _ip, _dp]),
+}
+
+_lib = None
+
+
+class Bri17Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("bri17_amd error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """Load libbri17_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                "libbri17_amd.so is not built (%s); run `python -c \"import __graft_entry__ as g; "
+                "g.build()\"` from the repository root" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != BRI17_OK:
+        msg = lib().bri17_last_error()
+        raise Bri17Error(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,317 @@
+// Device-side FFT engine for gfx950: Stockham autosort passes on sequences
+// staged in LDS, radix 2/3/4/5/7/8/16 butterflies held in VGPRs.
+//
+// Replaces the FFTW c2c transforms of the reference apply harness
+// (tests/test_bri17.cpp:57,95, plans :119-127).  Conventions follow FFTW /
+// sphinx/theory.rst:60-73: forward = exp(-2 pi i k n / N), unnormalised;
+// inverse = exp(+2 pi i k n / N), unnormalised.
+//
+// A workgroup transforms many sequences at once.  Each sequence of length N
+// is handled by `tpt` threads; for a pass of radix R a thread owns
+// ceil((N/R)/tpt) butterflies.  A pass reads every butterfly's R inputs into
+// registers, applies twiddles and the radix-R DFT, crosses a barrier and
+// writes the R outputs in Stockham order (in place), so the result lands in
+// natural order with no bit reversal.  Twiddles come from a table of
+// exp(-2 pi i m / Nt) evaluated in fp64 on the host and rounded to T.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bri17_types.hpp"
+
+namespace b17 {
+
+template <typename T>
+struct cpx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> operator+(cpx<T> a, cpx<T> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator-(cpx<T> a, cpx<T> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> operator*(cpx<T> a, T s) {
+  return {a.x * s, a.y * s};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> conj(cpx<T> a) {
+  return {a.x, -a.y};
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename T>
+__device__ __forceinline__ cpx<T> rot(cpx<T> a) {
+  return INV ? cpx<T>{-a.y, a.x} : cpx<T>{a.y, -a.x};
+}
+// multiply by the constant exp(-+ i theta) given (c, s) = (cos, sin) theta
+template <bool INV, typename T>
+__device__ __forceinline__ cpx<T> crot(cpx<T> a, T c, T s) {
+  // forward: (c - i s) a ; inverse: (c + i s) a
+  return INV ? cpx<T>{c * a.x - s * a.y, c * a.y + s * a.x}
+             : cpx<T>{c * a.x + s * a.y, c * a.y - s * a.x};
+}
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv d) {
+  return (__umulhi(n, d.m) + n) >> d.s;
+}
+
+// ---------------------------------------------------------------- butterflies
+
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly2(cpx<T>& a0, cpx<T>& a1) {
+  cpx<T> t = a0 - a1;
+  a0 = a0 + a1;
+  a1 = t;
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly4(cpx<T>& a0, cpx<T>& a1, cpx<T>& a2,
+                                      cpx<T>& a3) {
+  cpx<T> t0 = a0 + a2, t1 = a0 - a2, t2 = a1 + a3, t3 = rot<INV>(a1 - a3);
+  a0 = t0 + t2;
+  a2 = t0 - t2;
+  a1 = t1 + t3;
+  a3 = t1 - t3;
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly3(cpx<T>* v) {
+  const T s = T(0.86602540378443864676372317075294);
+  cpx<T> t = v[1] + v[2];
+  cpx<T> m = v[0] - t * T(0.5);
+  cpx<T> d = rot<INV>(v[1] - v[2]) * s;
+  v[0] = v[0] + t;
+  v[1] = m + d;
+  v[2] = m - d;
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly5(cpx<T>* v) {
+  const T c1 = T(0.30901699437494742410229341718282);
+  const T c2 = T(-0.80901699437494742410229341718282);
+  const T s1 = T(0.95105651629515357211643933337938);
+  const T s2 = T(0.58778525229247312916870595463907);
+  cpx<T> t1 = v[1] + v[4], t2 = v[2] + v[3], t3 = v[1] - v[4], t4 = v[2] - v[3];
+  cpx<T> b1 = v[0] + t1 * c1 + t2 * c2;
+  cpx<T> b2 = v[0] + t1 * c2 + t2 * c1;
+  cpx<T> e1 = rot<INV>(t3 * s1 + t4 * s2);
+  cpx<T> e2 = rot<INV>(t3 * s2 - t4 * s1);
+  v[0] = v[0] + t1 + t2;
+  v[1] = b1 + e1;
+  v[4] = b1 - e1;
+  v[2] = b2 + e2;
+  v[3] = b2 - e2;
+}
+
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly7(cpx<T>* v) {
+  const T c1 = T(0.62348980185873353052500488400424);
+  const T c2 = T(-0.22252093395631440428890256449679);
+  const T c3 = T(-0.90096886790241912623610231950745);
+  const T s1 = T(0.78183148246802980870844452667406);
+  const T s2 = T(0.97492791218182360701813168299393);
+  const T s3 = T(0.43388373911755812047576833284836);
+  cpx<T> p1 = v[1] + v[6], p2 = v[2] + v[5], p3 = v[3] + v[4];
+  cpx<T> m1 = v[1] - v[6], m2 = v[2] - v[5], m3 = v[3] - v[4];
+  cpx<T> b1 = v[0] + p1 * c1 + p2 * c2 + p3 * c3;
+  cpx<T> b2 = v[0] + p1 * c2 + p2 * c3 + p3 * c1;
+  cpx<T> b3 = v[0] + p1 * c3 + p2 * c1 + p3 * c2;
+  cpx<T> e1 = rot<INV>(m1 * s1 + m2 * s2 + m3 * s3);
+  cpx<T> e2 = rot<INV>(m1 * s2 - m2 * s3 - m3 * s1);
+  cpx<T> e3 = rot<INV>(m1 * s3 - m2 * s1 + m3 * s2);
+  v[0] = v[0] + p1 + p2 + p3;
+  v[1] = b1 + e1;
+  v[6] = b1 - e1;
+  v[2] = b2 + e2;
+  v[5] = b2 - e2;
+  v[3] = b3 + e3;
+  v[4] = b3 - e3;
+}
+
+// radix 8 = 2 x 4: DFT4 on even / odd samples, twiddle w8^k, DFT2.
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly8(cpx<T>* v) {
+  const T c = T(0.70710678118654752440084436210485);
+  bfly4<INV>(v[0], v[2], v[4], v[6]);
+  bfly4<INV>(v[1], v[3], v[5], v[7]);
+  cpx<T> o1 = crot<INV>(v[3], c, c);
+  cpx<T> o2 = rot<INV>(v[5]);
+  cpx<T> o3 = crot<INV>(v[7], -c, c);
+  cpx<T> y[8];
+  y[0] = v[0] + v[1];
+  y[4] = v[0] - v[1];
+  y[1] = v[2] + o1;
+  y[5] = v[2] - o1;
+  y[2] = v[4] + o2;
+  y[6] = v[4] - o2;
+  y[3] = v[6] + o3;
+  y[7] = v[6] - o3;
+#pragma unroll
+  for (int i = 0; i < 8; i++) v[i] = y[i];
+}
+
+// radix 16 = 4 x 4: DFT4 over q of x[4q+p], twiddle w16^(p ka), DFT4 over p.
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly16(cpx<T>* v) {
+  const T c1 = T(0.92387953251128675612818318939679);  // cos(pi/8)
+  const T s1 = T(0.38268343236508977172845998403040);  // sin(pi/8)
+  const T c2 = T(0.70710678118654752440084436210485);  // cos(pi/4)
+#pragma unroll
+  for (int p = 0; p < 4; p++) bfly4<INV>(v[p], v[p + 4], v[p + 8], v[p + 12]);
+  // Y[p][ka] sits at v[p + 4 ka]; twiddle by w16^(p ka)
+  v[5] = crot<INV>(v[5], c1, s1);     // p1 ka1: e=1
+  v[9] = crot<INV>(v[9], c2, c2);     // p1 ka2: e=2
+  v[13] = crot<INV>(v[13], s1, c1);   // p1 ka3: e=3
+  v[6] = crot<INV>(v[6], c2, c2);     // p2 ka1: e=2
+  v[10] = rot<INV>(v[10]);            // p2 ka2: e=4
+  v[14] = crot<INV>(v[14], -c2, c2);  // p2 ka3: e=6
+  v[7] = crot<INV>(v[7], s1, c1);     // p3 ka1: e=3
+  v[11] = crot<INV>(v[11], -c2, c2);  // p3 ka2: e=6
+  v[15] = crot<INV>(v[15], -c1, -s1); // p3 ka3: e=9
+  cpx<T> y[16];
+#pragma unroll
+  for (int ka = 0; ka < 4; ka++) {
+    cpx<T> a0 = v[4 * ka], a1 = v[4 * ka + 1], a2 = v[4 * ka + 2], a3 = v[4 * ka + 3];
+    bfly4<INV>(a0, a1, a2, a3);
+    y[ka] = a0;
+    y[ka + 4] = a1;
+    y[ka + 8] = a2;
+    y[ka + 12] = a3;
+  }
+#pragma unroll
+  for (int i = 0; i < 16; i++) v[i] = y[i];
+}
+
+template <int R, bool INV, typename T>
+__device__ __forceinline__ void bfly(cpx<T>* v) {
+  if constexpr (R == 2)
+    bfly2<INV>(v[0], v[1]);
+  else if constexpr (R == 3)
+    bfly3<INV>(v);
+  else if constexpr (R == 4)
+    bfly4<INV>(v[0], v[1], v[2], v[3]);
+  else if constexpr (R == 5)
+    bfly5<INV>(v);
+  else if constexpr (R == 7)
+    bfly7<INV>(v);
+  else if constexpr (R == 8)
+    bfly8<INV>(v);
+  else if constexpr (R == 16)
+    bfly16<INV>(v);
+}
+
+// ------------------------------------------------------------- LDS sequences
+
+// Where sequence s of a workgroup tile lives in LDS: element n of sequence s
+// is at base(s) + n * es (units of complex elements).
+struct SeqLayout {
+  int es;        // element stride
+  int w;         // sequences per "row group"
+  int gstride;   // LDS distance between row groups
+  FastDiv wdiv;  // division by w
+  __device__ __forceinline__ int base(int s) const {
+    uint32_t g = fdiv((uint32_t)s, wdiv);
+    return (int)g * gstride + (s - (int)g * w);
+  }
+};
+
+// One Stockham pass of radix R over the sequences this thread works on.
+// `lane` in [0, tpt) indexes the thread within its sequence.
+template <typename T, int R, bool INV, int MAXB>
+__device__ __forceinline__ void fft_pass(cpx<T>* buf, const cpx<T>* tw,
+                                         const PassDesc& P, int N, int tws,
+                                         int base, int es, int lane, int tpt,
+                                         bool active) {
+  const int NR = P.nr;
+  const int Ns = P.ns;
+  const int twm = P.twm * tws;
+  cpx<T> v[MAXB][R];
+  int ob[MAXB];
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < MAXB; i++) {
+      const int j = lane + i * tpt;
+      if (j < NR) {
+        const cpx<T>* src = buf + base + j * es;
+#pragma unroll
+        for (int k = 0; k < R; k++) v[i][k] = src[k * NR * es];
+        const int q = (int)fdiv((uint32_t)j, P.nsdiv);
+        const int jm = j - q * Ns;
+        if (Ns > 1) {
+#pragma unroll
+          for (int k = 1; k < R; k++) {
+            cpx<T> w = tw[jm * k * twm];
+            v[i][k] = INV ? cmulc(v[i][k], w) : cmul(v[i][k], w);
+          }
+        }
+        bfly<R, INV>(v[i]);
+        ob[i] = base + (q * Ns * R + jm) * es;
+      }
+    }
+  }
+  __syncthreads();
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < MAXB; i++) {
+      const int j = lane + i * tpt;
+      if (j < NR) {
+#pragma unroll
+        for (int k = 0; k < R; k++) buf[ob[i] + k * Ns * es] = v[i][k];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Full FFT of length D.n on the LDS sequence at `base` (stride es).  Every
+// thread of the workgroup must call this (barriers inside); `active` marks
+// threads that own a sequence.
+template <typename T, bool INV, int MAXE>
+__device__ __forceinline__ void lds_fft(cpx<T>* buf, const cpx<T>* tw,
+                                        const FftDesc& D, int base, int es,
+                                        int lane, bool active) {
+  for (int p = 0; p < D.npass; p++) {
+    const PassDesc& P = D.pass[p];
+    switch (P.r) {
+      case 2:
+        fft_pass<T, 2, INV, MAXE / 2>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 3:
+        fft_pass<T, 3, INV, MAXE / 3>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 4:
+        fft_pass<T, 4, INV, MAXE / 4>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 5:
+        fft_pass<T, 5, INV, MAXE / 5>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 7:
+        fft_pass<T, 7, INV, MAXE / 7>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 8:
+        fft_pass<T, 8, INV, MAXE / 8>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 16:
+        fft_pass<T, 16, INV, MAXE / 16>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+}  // namespace b17

@@ -1,0 +1,489 @@
+// HIP kernels of the matrix-free periodic stiffness operator (gfx950).
+//
+// Reference behaviour replaced: the FFT -> per-k K^(k).u^(k) -> inverse FFT
+// -> |h|/|N| scaling of StiffnessMatrixFactory::compute_Ku
+// (tests/test_bri17.cpp:56-107) with the modal stiffness of
+// Hooke::modal_stiffness (include/bri17/bri17.hpp:247-292), and the Green
+// solve with the k = 0 convention of sphinx/theory.rst:208-212.
+//
+// The transforms exploit that u is real and K^(-k) = K^(k): only the half
+// spectrum along the last axis is formed (R2C / C2R), which is exact.
+#include "fft_core.hpp"
+#include "kernels.hpp"
+
+namespace b17 {
+
+constexpr int kMaxE = 16;  // max complex elements per thread per FFT pass
+
+template <typename T>
+__device__ __forceinline__ cpx<T> post_r2c(cpx<T> A, cpx<T> B, cpx<T> w) {
+  // X[k] = (A + conj B)/2 - i w (A - conj B)/2, A = Z[k], B = Z[m-k]
+  cpx<T> e = {T(0.5) * (A.x + B.x), T(0.5) * (A.y - B.y)};
+  cpx<T> d = {T(0.5) * (A.x - B.x), T(0.5) * (A.y + B.y)};
+  cpx<T> o = {d.y, -d.x};  // -i d
+  return e + cmul(w, o);
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
+  // Z[k] = (A + conj B) + i conj(w) (A - conj B), A = X[k], B = X[m-k]
+  cpx<T> e = {A.x + B.x, A.y - B.y};
+  cpx<T> d = {A.x - B.x, A.y + B.y};
+  cpx<T> o = cmulc(d, w);
+  return {e.x - o.y, e.y + o.x};
+}
+
+// ------------------------------------------------------------------ plane fwd
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+    k_plane_fwd(const T* __restrict__ u, cpx<T>* __restrict__ ws, const PlaneParams pp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+  const cpx<T>* twr = reinterpret_cast<const cpx<T>*>(pp.tw_row);
+  const cpx<T>* twc = reinterpret_cast<const cpx<T>*>(pp.tw_col);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int plane = blockIdx.x / pp.p;
+  const int p = blockIdx.x - plane * pp.p;
+  int nrows = pp.q;
+  if (!pp.has_col) {
+    const int rem = pp.nrows - plane * pp.n1;
+    nrows = rem < pp.q ? rem : pp.q;
+  }
+  const int64_t row0 = (int64_t)plane * pp.n1 + p;
+  const int pst = pp.p;
+
+  // 1. rows -> LDS (even rows are read as m complex pairs)
+  if (pp.even) {
+    const int m = pp.m, tot = nrows * m;
+    for (int e = tid; e < tot; e += nthr) {
+      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+      const int mm = e - qq * m;
+      const cpx<T>* src =
+          reinterpret_cast<const cpx<T>*>(u + (row0 + (int64_t)qq * pst) * pp.nl);
+      lds[qq * pp.rsl + mm] = src[mm];
+    }
+  } else {
+    const int m = pp.m, tot = nrows * m;
+    for (int e = tid; e < tot; e += nthr) {
+      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+      const int mm = e - qq * m;
+      lds[qq * pp.rsl + mm] = {u[(row0 + (int64_t)qq * pst) * pp.nl + mm], T(0)};
+    }
+  }
+  __syncthreads();
+
+  // 2. row FFTs, consecutive threads on consecutive elements of a row
+  {
+    const int tpt = pp.row.tpt, S = nthr / tpt;
+    const int sl = tid / tpt, lane = tid - sl * tpt;
+    for (int r0 = 0; r0 < nrows; r0 += S) {
+      const int s = r0 + sl;
+      const bool act = sl < S && s < nrows;
+      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, lane, act);
+    }
+  }
+
+  // 3. split the packed transform into the nl/2+1 spectrum entries
+  if (pp.even) {
+    const int m = pp.m, h = m / 2 + 1, tot = nrows * h;
+    for (int e = tid; e < tot; e += nthr) {
+      const int qq = (int)fdiv((uint32_t)e, pp.hdiv);
+      const int k = e - qq * h;
+      cpx<T>* row = lds + qq * pp.rsl;
+      if (k == 0) {
+        const cpx<T> z = row[0];
+        row[0] = {z.x + z.y, T(0)};
+        row[m] = {z.x - z.y, T(0)};
+      } else {
+        const int k2 = m - k;
+        const cpx<T> A = row[k], B = row[k2];
+        row[k] = post_r2c(A, B, twr[k]);
+        if (k2 != k) row[k2] = post_r2c(B, A, twr[k2]);
+      }
+    }
+    __syncthreads();
+  }
+
+  // 4. 3D: Q-point stage of the middle axis on every column, then the split
+  //    twiddle w_N1^(p ka) is applied on the way out
+  if (pp.has_col) {
+    const int tpt = pp.col.tpt, S = nthr / tpt;
+    const int sl = tid % S, lane = tid / S;
+    const bool inr = tid < S * tpt;
+    for (int c0 = 0; c0 < pp.nlh; c0 += S) {
+      const int s = c0 + sl;
+      const bool act = inr && s < pp.nlh;
+      lds_fft<T, false, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, lane, act);
+    }
+  }
+
+  // 5. store
+  {
+    const int tot = nrows * pp.nlh;
+    for (int e = tid; e < tot; e += nthr) {
+      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
+      const int k = e - ka * pp.nlh;
+      cpx<T> v = lds[ka * pp.rsl + k];
+      if (p > 0) v = cmul(v, twc[p * ka]);
+      ws[((int64_t)plane * pp.n1 + p * pp.q + ka) * pp.rs + k] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ plane inv
+
+template <typename T>
+__global__ void __launch_bounds__(1024)
+    k_plane_inv(const cpx<T>* __restrict__ ws, T* __restrict__ out, const PlaneParams pp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+  const cpx<T>* twr = reinterpret_cast<const cpx<T>*>(pp.tw_row);
+  const cpx<T>* twc = reinterpret_cast<const cpx<T>*>(pp.tw_col);
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int plane = blockIdx.x / pp.p;
+  const int p = blockIdx.x - plane * pp.p;
+  int nrows = pp.q;
+  if (!pp.has_col) {
+    const int rem = pp.nrows - plane * pp.n1;
+    nrows = rem < pp.q ? rem : pp.q;
+  }
+  const int64_t row0 = (int64_t)plane * pp.n1 + p;
+  const int pst = pp.p;
+
+  // 1. load (undo the split twiddle)
+  {
+    const int tot = nrows * pp.nlh;
+    for (int e = tid; e < tot; e += nthr) {
+      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
+      const int k = e - ka * pp.nlh;
+      cpx<T> v = ws[((int64_t)plane * pp.n1 + p * pp.q + ka) * pp.rs + k];
+      if (p > 0) v = cmulc(v, twc[p * ka]);
+      lds[ka * pp.rsl + k] = v;
+    }
+  }
+  __syncthreads();
+
+  // 2. 3D: inverse Q-point stage of the middle axis
+  if (pp.has_col) {
+    const int tpt = pp.col.tpt, S = nthr / tpt;
+    const int sl = tid % S, lane = tid / S;
+    const bool inr = tid < S * tpt;
+    for (int c0 = 0; c0 < pp.nlh; c0 += S) {
+      const int s = c0 + sl;
+      const bool act = inr && s < pp.nlh;
+      lds_fft<T, true, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, lane, act);
+    }
+  }
+
+  // 3. rebuild the transform the row FFT needs
+  if (pp.even) {
+    const int m = pp.m, h = m / 2 + 1, tot = nrows * h;
+    for (int e = tid; e < tot; e += nthr) {
+      const int qq = (int)fdiv((uint32_t)e, pp.hdiv);
+      const int k = e - qq * h;
+      cpx<T>* row = lds + qq * pp.rsl;
+      if (k == 0) {
+        const cpx<T> A = row[0], B = row[m];
+        const cpx<T> ee = {A.x + B.x, A.y - B.y};
+        const cpx<T> d = {A.x - B.x, A.y + B.y};
+        row[0] = {ee.x - d.y, ee.y + d.x};
+      } else {
+        const int k2 = m - k;
+        const cpx<T> A = row[k], B = row[k2];
+        row[k] = pre_c2r(A, B, twr[k]);
+        if (k2 != k) row[k2] = pre_c2r(B, A, twr[k2]);
+      }
+    }
+  } else {
+    // odd length: Hermitian completion X[n] = conj X[nl - n]
+    const int nl = pp.nl, nh = pp.nlh, ext = nl - nh, tot = nrows * ext;
+    for (int e = tid; e < tot; e += nthr) {
+      const int qq = e / ext;
+      const int n = nh + (e - qq * ext);
+      cpx<T>* row = lds + qq * pp.rsl;
+      row[n] = conj(row[nl - n]);
+    }
+  }
+  __syncthreads();
+
+  // 4. inverse row FFTs
+  {
+    const int tpt = pp.row.tpt, S = nthr / tpt;
+    const int sl = tid / tpt, lane = tid - sl * tpt;
+    for (int r0 = 0; r0 < nrows; r0 += S) {
+      const int s = r0 + sl;
+      const bool act = sl < S && s < nrows;
+      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, lane, act);
+    }
+  }
+
+  // 5. store real rows
+  {
+    const int m = pp.m, tot = nrows * m;
+    for (int e = tid; e < tot; e += nthr) {
+      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+      const int mm = e - qq * m;
+      if (pp.even) {
+        cpx<T>* dst = reinterpret_cast<cpx<T>*>(out + (row0 + (int64_t)qq * pst) * pp.nl);
+        dst[mm] = lds[qq * pp.rsl + mm];
+      } else {
+        out[(row0 + (int64_t)qq * pst) * pp.nl + mm] = lds[qq * pp.rsl + mm].x;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------- pencil
+
+template <typename T, int R, bool INV>
+__device__ __forceinline__ void pdft_r(cpx<T>* lds, int W, FastDiv wdiv, int items,
+                                       int tid, int nthr) {
+  for (int e = tid; e < items; e += nthr) {
+    const int outer = (int)fdiv((uint32_t)e, wdiv);
+    const int w = e - outer * W;
+    cpx<T>* b = lds + outer * R * W + w;
+    cpx<T> v[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) v[k] = b[k * W];
+    bfly<R, INV>(v);
+#pragma unroll
+    for (int k = 0; k < R; k++) b[k * W] = v[k];
+  }
+}
+
+// P-point DFT over the split index p for every (c, i0, w) of the tile.
+template <typename T, bool INV>
+__device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
+                                     int nouter, int tid, int nthr) {
+  const int items = nouter * W;
+  switch (P) {
+    case 2: pdft_r<T, 2, INV>(lds, W, wdiv, items, tid, nthr); break;
+    case 3: pdft_r<T, 3, INV>(lds, W, wdiv, items, tid, nthr); break;
+    case 4: pdft_r<T, 4, INV>(lds, W, wdiv, items, tid, nthr); break;
+    case 5: pdft_r<T, 5, INV>(lds, W, wdiv, items, tid, nthr); break;
+    case 7: pdft_r<T, 7, INV>(lds, W, wdiv, items, tid, nthr); break;
+    case 8: pdft_r<T, 8, INV>(lds, W, wdiv, items, tid, nthr); break;
+    case 16: pdft_r<T, 16, INV>(lds, W, wdiv, items, tid, nthr); break;
+    default: break;
+  }
+  __syncthreads();
+}
+
+// The modal DIM x DIM block at frequency k (bri17.hpp:255-288), applied to x
+// (MODE 0: y = scale K^ x) or solved against x (MODE 1: y = scale K^-1 x,
+// y = 0 at k = 0).  K^ is real symmetric.
+template <typename T, int DIM, int MODE>
+__device__ __forceinline__ void modal_block(const ModalCoef<T>* const* coef, int k0,
+                                            int k1, int kl, T mu, T s, T scale,
+                                            cpx<T>* x) {
+  if constexpr (DIM == 2) {
+    const ModalCoef<T> a = coef[0][k0], b = coef[1][kl];
+    const T H0 = a.phi * b.chi, H1 = a.chi * b.phi;
+    const T dg = mu * (H0 + H1);
+    const T K00 = s * H0 + dg, K11 = s * H1 + dg, K01 = s * a.psi * b.psi;
+    if constexpr (MODE == 0) {
+      const cpx<T> y0 = (x[0] * K00 + x[1] * K01) * scale;
+      const cpx<T> y1 = (x[0] * K01 + x[1] * K11) * scale;
+      x[0] = y0;
+      x[1] = y1;
+    } else {
+      if (k0 == 0 && kl == 0) {
+        x[0] = {T(0), T(0)};
+        x[1] = {T(0), T(0)};
+        return;
+      }
+      const T r = scale / (K00 * K11 - K01 * K01);
+      const cpx<T> y0 = (x[0] * K11 - x[1] * K01) * r;
+      const cpx<T> y1 = (x[1] * K00 - x[0] * K01) * r;
+      x[0] = y0;
+      x[1] = y1;
+    }
+  } else {
+    const ModalCoef<T> a = coef[0][k0], b = coef[1][k1], c = coef[2][kl];
+    const T H0 = a.phi * b.chi * c.chi;
+    const T H1 = a.chi * b.phi * c.chi;
+    const T H2 = a.chi * b.chi * c.phi;
+    const T dg = mu * (H0 + H1 + H2);
+    const T K00 = s * H0 + dg, K11 = s * H1 + dg, K22 = s * H2 + dg;
+    const T K01 = s * a.psi * b.psi * c.chi;
+    const T K02 = s * a.psi * b.chi * c.psi;
+    const T K12 = s * a.chi * b.psi * c.psi;
+    if constexpr (MODE == 0) {
+      const cpx<T> y0 = (x[0] * K00 + x[1] * K01 + x[2] * K02) * scale;
+      const cpx<T> y1 = (x[0] * K01 + x[1] * K11 + x[2] * K12) * scale;
+      const cpx<T> y2 = (x[0] * K02 + x[1] * K12 + x[2] * K22) * scale;
+      x[0] = y0;
+      x[1] = y1;
+      x[2] = y2;
+    } else {
+      if (k0 == 0 && k1 == 0 && kl == 0) {
+        x[0] = x[1] = x[2] = {T(0), T(0)};
+        return;
+      }
+      // adjugate of the symmetric SPD block
+      const T A00 = K11 * K22 - K12 * K12;
+      const T A01 = K02 * K12 - K01 * K22;
+      const T A02 = K01 * K12 - K02 * K11;
+      const T A11 = K00 * K22 - K02 * K02;
+      const T A12 = K01 * K02 - K00 * K12;
+      const T A22 = K00 * K11 - K01 * K01;
+      const T r = scale / (K00 * A00 + K01 * A01 + K02 * A02);
+      const cpx<T> y0 = (x[0] * A00 + x[1] * A01 + x[2] * A02) * r;
+      const cpx<T> y1 = (x[0] * A01 + x[1] * A11 + x[2] * A12) * r;
+      const cpx<T> y2 = (x[0] * A02 + x[1] * A12 + x[2] * A22) * r;
+      x[0] = y0;
+      x[1] = y1;
+      x[2] = y2;
+    }
+  }
+}
+
+template <typename T, int DIM, int MODE>
+__global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const PencilParams pp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
+  const cpx<T>* tw0 = reinterpret_cast<const cpx<T>*>(pp.tw0);
+  const ModalCoef<T>* coef[3] = {reinterpret_cast<const ModalCoef<T>*>(pp.coef[0]),
+                                 reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]),
+                                 reinterpret_cast<const ModalCoef<T>*>(pp.coef[2])};
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  int t = blockIdx.x;
+  if (pp.xcd_remap) {
+    // consecutive tiles share L2 lines: keep them on one XCD (blocks are
+    // dealt round-robin over the 8 XCDs)
+    const int n = gridDim.x, per = n / 8, rem = n % 8;
+    const int x = t % 8, idx = t / 8;
+    t = x * per + (x < rem ? x : rem) + idx;
+  }
+  const int W = pp.w, P = pp.p, N0 = pp.n0, PW = P * W;
+  const int f0 = t * W;
+  const int wv = pp.f - f0 < W ? pp.f - f0 : W;
+  const int tot = DIM * N0 * PW;
+  const int cst = N0 * PW;  // LDS distance between components
+
+  // load the tile: element (c, i0, p, w) at ((c N0 + i0) P + p) W + w
+  for (int e = tid; e < tot; e += nthr) {
+    const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
+    const int w = e - rest * W;
+    cpx<T> v = {T(0), T(0)};
+    if (w < wv) v = ws[(int64_t)rest * pp.f + f0 + w];
+    lds[e] = v;
+  }
+  __syncthreads();
+  if (P > 1) pdft<T, false>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+
+  const int tpt = pp.ax0.tpt, S = nthr / tpt;
+  const int sl = tid % S, lane = tid / S;
+  const bool inr = tid < S * tpt;
+  const int nseq = DIM * PW;
+  for (int s0 = 0; s0 < nseq; s0 += S) {
+    const int s = s0 + sl;
+    const bool act = inr && s < nseq;
+    const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
+    const int base = act ? g * cst + (s - g * PW) : 0;
+    lds_fft<T, false, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
+  }
+
+  {
+    const T mu = T(pp.mu), sc = T(pp.s), scale = T(pp.scale);
+    const int items = N0 * PW;
+    for (int e = tid; e < items; e += nthr) {
+      const int k0 = (int)fdiv((uint32_t)e, pp.pwdiv);
+      const int r = e - k0 * PW;
+      const int kb = (int)fdiv((uint32_t)r, pp.wdiv);
+      const int w = r - kb * W;
+      if (w >= wv) continue;
+      const int f = f0 + w;
+      const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
+      const int kl = f - ka * pp.rs;
+      if (kl >= pp.nlh) continue;
+      const int k1 = ka + pp.q * kb;
+      cpx<T> x[DIM];
+#pragma unroll
+      for (int c = 0; c < DIM; c++) x[c] = lds[c * cst + e];
+      modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, scale, x);
+#pragma unroll
+      for (int c = 0; c < DIM; c++) lds[c * cst + e] = x[c];
+    }
+  }
+  __syncthreads();
+
+  for (int s0 = 0; s0 < nseq; s0 += S) {
+    const int s = s0 + sl;
+    const bool act = inr && s < nseq;
+    const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
+    const int base = act ? g * cst + (s - g * PW) : 0;
+    lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
+  }
+  if (P > 1) pdft<T, true>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+
+  for (int e = tid; e < tot; e += nthr) {
+    const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
+    const int w = e - rest * W;
+    if (w < wv) ws[(int64_t)rest * pp.f + f0 + w] = lds[e];
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+
+template <typename K>
+static hipError_t set_lds(K kernel, size_t lds) {
+  if (lds > 65536)
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp, int nblocks,
+                            int nthreads, size_t lds, hipStream_t st) {
+  hipError_t e = set_lds(k_plane_fwd<T>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_plane_fwd<T>, dim3(nblocks), dim3(nthreads), lds, st, u,
+                     reinterpret_cast<cpx<T>*>(ws), pp);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_plane_inv(const void* ws, T* out, const PlaneParams& pp, int nblocks,
+                            int nthreads, size_t lds, hipStream_t st) {
+  hipError_t e = set_lds(k_plane_inv<T>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_plane_inv<T>, dim3(nblocks), dim3(nthreads), lds, st,
+                     reinterpret_cast<const cpx<T>*>(ws), out, pp);
+  return hipGetLastError();
+}
+
+template <typename T, int DIM, int MODE>
+static hipError_t launch_pencil_t(void* ws, const PencilParams& pp, int nthreads, size_t lds,
+                                  hipStream_t st) {
+  hipError_t e = set_lds(k_pencil<T, DIM, MODE>, lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_pencil<T, DIM, MODE>), dim3(pp.ntiles), dim3(nthreads), lds, st,
+                     reinterpret_cast<cpx<T>*>(ws), pp);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_pencil(void* ws, const PencilParams& pp, int nthreads, size_t lds,
+                         hipStream_t st) {
+  if (pp.dim == 2)
+    return pp.mode == 0 ? launch_pencil_t<T, 2, 0>(ws, pp, nthreads, lds, st)
+                        : launch_pencil_t<T, 2, 1>(ws, pp, nthreads, lds, st);
+  return pp.mode == 0 ? launch_pencil_t<T, 3, 0>(ws, pp, nthreads, lds, st)
+                      : launch_pencil_t<T, 3, 1>(ws, pp, nthreads, lds, st);
+}
+
+template hipError_t launch_plane_fwd<float>(const float*, void*, const PlaneParams&, int, int,
+                                            size_t, hipStream_t);
+template hipError_t launch_plane_fwd<double>(const double*, void*, const PlaneParams&, int, int,
+                                             size_t, hipStream_t);
+template hipError_t launch_plane_inv<float>(const void*, float*, const PlaneParams&, int, int,
+                                            size_t, hipStream_t);
+template hipError_t launch_plane_inv<double>(const void*, double*, const PlaneParams&, int, int,
+                                             size_t, hipStream_t);
+template hipError_t launch_pencil<float>(void*, const PencilParams&, int, size_t, hipStream_t);
+template hipError_t launch_pencil<double>(void*, const PencilParams&, int, size_t, hipStream_t);
+
+}  // namespace b17

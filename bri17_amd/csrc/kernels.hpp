@@ -1,0 +1,84 @@
+// Kernel parameter blocks and launchers of the periodic stiffness operator.
+//
+// Pipeline for a field u of shape (DIM, N0, ..., NL) in C order (the layout of
+// tests/test_bri17.cpp:66-67,81-83):
+//
+//   plane_fwd  : R2C along the last axis (+ in 3D the first Cooley-Tukey stage
+//                of the middle axis), writes the half-spectrum workspace
+//   pencil     : remaining middle-axis stage, FFT along axis 0, the modal
+//                DIM x DIM block (K^ multiply or K^-1 solve, bri17.hpp:247-292),
+//                inverse FFT along axis 0, inverse middle-axis stage
+//   plane_inv  : inverse middle-axis stage + C2R along the last axis
+//
+// Workspace layout (complex T): ws[c][i0][p][ka][k] with middle index
+// i1 = p*Q + ka (3D) and k in [0, rs), rs = NL/2+1; 2D: ws[c][i0][k].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bri17_types.hpp"
+
+namespace b17 {
+
+// Per-axis modal coefficient table entry (bri17.hpp:255-264), precomputed in
+// fp64 on the host with the half-angle form phi = 4 sin^2(pi k/N)/h^2.
+template <typename T>
+struct ModalCoef {
+  T phi, chi, psi, pad;
+};
+
+struct PlaneParams {
+  int dim;
+  int nl;        // row length (last axis)
+  int nlh;       // nl/2 + 1 stored spectrum entries per row
+  int even;      // nl even: packed half-length complex FFT
+  int m;         // row FFT length: nl/2 (even) or nl (odd)
+  int rsl;       // LDS row stride (complex elements)
+  int q;         // rows per workgroup
+  int p;         // middle-axis split factor (1 in 2D)
+  int n1;        // 3D: middle axis length; 2D: rows per workgroup
+  int has_col;   // 3D: do the Q-point middle-axis stage
+  int rs;        // workspace row stride (complex elements)
+  int nrows;     // 2D: total number of rows (DIM*N0); 3D unused
+  FastDiv mdiv;  // division by m
+  FastDiv hdiv;  // division by m/2+1 (pairs per row)
+  FastDiv nlhdiv;
+  FftDesc row;   // length m
+  FftDesc col;   // length q (3D)
+  const void* tw_row;  // exp(-2 pi i j / nl), j < nl
+  const void* tw_col;  // exp(-2 pi i j / n1), j < n1 (3D)
+};
+
+struct PencilParams {
+  int dim;
+  int n0;
+  int p;        // middle-axis split factor
+  int q;        // N1 / p (3D); 1 (2D)
+  int rs;       // workspace row stride
+  int nlh;      // valid entries per row
+  int f;        // flat extent of one p-block: q * rs
+  int w;        // tile width (contiguous complex elements)
+  int ntiles;
+  int mode;     // 0: K u, 1: K^-1 f
+  int xcd_remap;
+  double mu, s, scale;
+  FastDiv wdiv, pwdiv, rsdiv;
+  FftDesc ax0;
+  const void* tw0;      // exp(-2 pi i j / n0)
+  const void* coef[3];  // ModalCoef<T>[N_d] per axis
+};
+
+template <typename T>
+hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp,
+                            int nblocks, int nthreads, size_t lds,
+                            hipStream_t st);
+template <typename T>
+hipError_t launch_plane_inv(const void* ws, T* out, const PlaneParams& pp,
+                            int nblocks, int nthreads, size_t lds,
+                            hipStream_t st);
+template <typename T>
+hipError_t launch_pencil(void* ws, const PencilParams& pp, int nthreads,
+                         size_t lds, hipStream_t st);
+
+}  // namespace b17

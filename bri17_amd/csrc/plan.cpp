@@ -1,0 +1,317 @@
+// Host-side planning of the periodic stiffness operator.
+//
+// The reference builds one FFTW c2c plan per component at construction
+// (tests/test_bri17.cpp:110-128) and recomputes the modal stiffness with libm
+// trigonometry at every frequency (bri17.hpp:255-264).  Here a plan holds, per
+// grid, device-resident twiddle tables and per-axis modal coefficient tables
+// (phi, chi, psi) evaluated once in extended precision on the host, plus the
+// launch geometry of the three kernels.  Tables depend on (shape, L) only;
+// mu and nu enter the kernels as scalars.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace b17 {
+
+static thread_local std::string t_err;
+void set_error(const std::string& msg) { t_err = msg; }
+const std::string& get_error() { return t_err; }
+
+static Status fail(int code, const std::string& m) { return Status{code, m}; }
+
+std::vector<int> default_radices(int n) {
+  std::vector<int> r;
+  if (n < 1) return r;
+  while (n % 16 == 0) r.push_back(16), n /= 16;
+  if (n % 8 == 0) r.push_back(8), n /= 8;
+  if (n % 4 == 0) r.push_back(4), n /= 4;
+  if (n % 2 == 0) r.push_back(2), n /= 2;
+  for (int p : {3, 5, 7})
+    while (n % p == 0) r.push_back(p), n /= p;
+  if (n != 1) r.clear();
+  return r;
+}
+
+bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
+                   std::string* why) {
+  std::memset(&d, 0, sizeof(d));
+  d.n = n;
+  d.tws = tws;
+  if (n == 1) {
+    d.npass = 0;
+    d.tpt = 1;
+    return true;
+  }
+  int prod = 1, rmax = 1;
+  for (int r : radices) {
+    if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16)) {
+      if (why) *why = "unsupported radix " + std::to_string(r);
+      return false;
+    }
+    prod *= r;
+    rmax = std::max(rmax, r);
+  }
+  if (prod != n || radices.empty() || (int)radices.size() > kMaxPass) {
+    if (why) *why = "radix order does not factor length " + std::to_string(n);
+    return false;
+  }
+  // threads per sequence: every pass must fit kMaxE elements per thread
+  constexpr int kMaxE = 16;
+  int tpt = n / rmax;
+  for (;; tpt++) {
+    bool ok = true;
+    for (int r : radices) {
+      int nr = n / r;
+      int per = (nr + tpt - 1) / tpt;
+      if (per * r > kMaxE) ok = false;
+    }
+    if (ok) break;
+  }
+  d.tpt = tpt;
+  d.npass = (int)radices.size();
+  int ns = 1;
+  for (int i = 0; i < d.npass; i++) {
+    PassDesc& p = d.pass[i];
+    p.r = radices[i];
+    p.ns = ns;
+    p.nr = n / p.r;
+    p.twm = n / (ns * p.r);
+    p.nsdiv = make_fastdiv((uint32_t)ns);
+    ns *= p.r;
+  }
+  return true;
+}
+
+Plan::~Plan() {
+  for (int d = 0; d < 3; d++) {
+    if (tw[d]) (void)hipFree(tw[d]);
+    if (coef[d]) (void)hipFree(coef[d]);
+  }
+}
+
+template <typename T>
+static Status upload_tables(Plan& pl) {
+  const long double PI = 3.141592653589793238462643383279502884L;
+  for (int d = 0; d < pl.dim; d++) {
+    const int n = pl.shape[d];
+    std::vector<T> tw(2 * (size_t)n);
+    for (int j = 0; j < n; j++) {
+      long double a = 2.0L * PI * (long double)j / (long double)n;
+      tw[2 * j] = (T)cosl(a);
+      tw[2 * j + 1] = (T)(-sinl(a));
+    }
+    // phi = 2(1-cos b)/h^2 = 4 sin^2(b/2)/h^2, chi = (2+cos b)/3,
+    // psi = sin b / h with b = 2 pi k / N  (bri17.hpp:258-264)
+    std::vector<T> cf(4 * (size_t)n);
+    const long double h = (long double)pl.L[d] / (long double)n;
+    for (int k = 0; k < n; k++) {
+      long double sh = sinl(PI * (long double)k / (long double)n);
+      long double b = 2.0L * PI * (long double)k / (long double)n;
+      cf[4 * k + 0] = (T)(4.0L * sh * sh / (h * h));
+      cf[4 * k + 1] = (T)(1.0L - 2.0L * sh * sh / 3.0L);
+      cf[4 * k + 2] = (T)(sinl(b) / h);
+      cf[4 * k + 3] = (T)0;
+    }
+    hipError_t e = hipMalloc(&pl.tw[d], tw.size() * sizeof(T));
+    if (e != hipSuccess) return fail(3, std::string("hipMalloc(twiddles): ") + hipGetErrorString(e));
+    e = hipMalloc(&pl.coef[d], cf.size() * sizeof(T));
+    if (e != hipSuccess) return fail(3, std::string("hipMalloc(modal table): ") + hipGetErrorString(e));
+    e = hipMemcpy(pl.tw[d], tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+      e = hipMemcpy(pl.coef[d], cf.data(), cf.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(3, std::string("hipMemcpy(tables): ") + hipGetErrorString(e));
+  }
+  return Status{};
+}
+
+static int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+static const std::vector<int>& pick(const Tuning& t, int role, const std::vector<int>& dflt) {
+  return t.radices[role].empty() ? dflt : t.radices[role];
+}
+
+Status build_plan(int dim, const int* shape, const double* L, int dtype, const Tuning& tun,
+                  std::shared_ptr<Plan>& out) {
+  if (dim != 2 && dim != 3) return fail(1, "dim must be 2 or 3");
+  if (dtype != 0 && dtype != 1) return fail(1, "dtype must be BRI17_F32 or BRI17_F64");
+  auto pl = std::make_shared<Plan>();
+  pl->dim = dim;
+  pl->dtype = dtype;
+  pl->esz = dtype == 0 ? 4 : 8;
+  pl->tuning = tun;
+  int64_t nn = 1;
+  double hv = 1.0;
+  for (int d = 0; d < dim; d++) {
+    if (shape[d] < 1) return fail(1, "grid shape must be positive");
+    if (!(L[d] > 0.0) || !std::isfinite(L[d])) return fail(1, "grid lengths must be positive and finite");
+    if (default_radices(shape[d]).empty() && shape[d] != 1)
+      return fail(2, "grid side " + std::to_string(shape[d]) + " has a prime factor > 7");
+    pl->shape[d] = shape[d];
+    pl->L[d] = L[d];
+    nn *= shape[d];
+    hv *= L[d] / shape[d];
+  }
+  if ((int64_t)dim * nn >= (int64_t(1) << 31))
+    return fail(1, "DIM*|N| must be < 2^31 (the reference's int sizes, bri17.hpp:48)");
+  pl->nnodes = nn;
+  pl->cell_volume = hv;
+  const size_t csz = 2 * pl->esz;  // complex element
+  const size_t budget = std::min<size_t>(tun.lds_budget, 160 * 1024);
+
+  const int nl = shape[dim - 1];
+  const int even = nl % 2 == 0;
+  const int m = even ? nl / 2 : nl;
+  const int nlh = nl / 2 + 1;
+  const int rsl = even ? m + 1 : nl;
+  std::string why;
+
+  PlaneParams& pp = pl->plane;
+  std::memset(&pp, 0, sizeof(pp));
+  pp.dim = dim;
+  pp.nl = nl;
+  pp.nlh = nlh;
+  pp.even = even;
+  pp.m = m;
+  pp.rsl = rsl;
+  pp.rs = nlh;
+  if (!make_fft_desc(m, nl / m, pick(tun, 0, default_radices(m)), pp.row, &why))
+    return fail(2, "row FFT: " + why);
+  pp.mdiv = make_fastdiv((uint32_t)m);
+  pp.hdiv = make_fastdiv((uint32_t)(m / 2 + 1));
+  pp.nlhdiv = make_fastdiv((uint32_t)nlh);
+
+  PencilParams& pc = pl->pencil;
+  std::memset(&pc, 0, sizeof(pc));
+  pc.dim = dim;
+  pc.n0 = shape[0];
+  pc.rs = nlh;
+  pc.nlh = nlh;
+  pc.xcd_remap = tun.xcd_remap;
+
+  if (dim == 3) {
+    const int n1 = shape[1];
+    // smallest split P of the middle axis whose Q = N1/P rows fit in LDS
+    int P = 0;
+    const int cands[] = {1, 2, 3, 4, 5, 7, 8, 16};
+    for (int c : cands) {
+      if (tun.plane_split && c != tun.plane_split) continue;
+      if (n1 % c) continue;
+      if ((size_t)(n1 / c) * rsl * csz <= budget) {
+        P = c;
+        break;
+      }
+    }
+    if (!P) return fail(2, "no middle-axis split fits the LDS budget for this grid");
+    const int Q = n1 / P;
+    pp.q = Q;
+    pp.p = P;
+    pp.n1 = n1;
+    pp.has_col = 1;
+    if (!make_fft_desc(Q, P, pick(tun, 1, default_radices(Q)), pp.col, &why))
+      return fail(2, "middle-axis FFT: " + why);
+    int nthr = std::max(Q * pp.row.tpt, nlh * pp.col.tpt);
+    nthr = std::min(1024, round_up(nthr, 64));
+    nthr = std::max(nthr, round_up(std::max(pp.row.tpt, pp.col.tpt), 64));
+    pl->plane_threads = nthr;
+    pl->plane_blocks = dim * shape[0] * P;
+    pl->plane_lds = (size_t)Q * rsl * csz;
+    pc.p = P;
+    pc.q = Q;
+    pc.f = Q * nlh;
+  } else {
+    const int nrows = dim * shape[0];
+    int G = std::max(1, std::min(256 / pp.row.tpt, (nrows + 255) / 256));
+    G = std::min<int>(G, (int)(budget / (rsl * csz)));
+    G = std::max(1, std::min(G, nrows));
+    pp.q = G;
+    pp.p = 1;
+    pp.n1 = G;
+    pp.has_col = 0;
+    pp.nrows = nrows;
+    std::memset(&pp.col, 0, sizeof(pp.col));
+    int nthr = std::min(1024, round_up(G * pp.row.tpt, 64));
+    pl->plane_threads = nthr;
+    pl->plane_blocks = (nrows + G - 1) / G;
+    pl->plane_lds = (size_t)G * rsl * csz;
+    pc.p = 1;
+    pc.q = 1;
+    pc.f = nlh;
+  }
+
+  // pencil tile width: widest power of two that fits the LDS budget
+  if (!make_fft_desc(shape[0], 1, pick(tun, 2, default_radices(shape[0])), pc.ax0, &why))
+    return fail(2, "axis-0 FFT: " + why);
+  const size_t per_w = (size_t)dim * shape[0] * pc.p * csz;
+  int W = tun.pencil_width;
+  if (!W) {
+    W = 1;
+    while (W < 64 && per_w * (2 * W) <= budget && 2 * W <= round_up(pc.f, 1)) W *= 2;
+  }
+  if (per_w * W > budget) return fail(2, "pencil tile does not fit the LDS budget for this grid");
+  pc.w = W;
+  pc.ntiles = (pc.f + W - 1) / W;
+  pc.wdiv = make_fastdiv((uint32_t)W);
+  pc.pwdiv = make_fastdiv((uint32_t)(pc.p * W));
+  pc.rsdiv = make_fastdiv((uint32_t)nlh);
+  {
+    int nseq = dim * pc.p * W;
+    int nthr = std::min(1024, round_up(nseq * pc.ax0.tpt, 64));
+    nthr = std::max(nthr, round_up(pc.ax0.tpt, 64));
+    pl->pencil_threads = nthr;
+    pl->pencil_lds = per_w * W;
+  }
+  pl->ws_bytes = (size_t)dim * (nn / nl) * nlh * csz;
+
+  Status s = dtype == 0 ? upload_tables<float>(*pl) : upload_tables<double>(*pl);
+  if (!s.ok()) return s;
+  pp.tw_row = pl->tw[dim - 1];
+  pp.tw_col = dim == 3 ? pl->tw[1] : nullptr;
+  pc.tw0 = pl->tw[0];
+  for (int d = 0; d < 3; d++) pc.coef[d] = d < dim ? pl->coef[d] : nullptr;
+  out = pl;
+  return Status{};
+}
+
+Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
+                    void* ws, hipStream_t st, hipEvent_t* ev) {
+  PencilParams pc = pl.pencil;
+  pc.mu = mu;
+  pc.s = mu / (1.0 - 2.0 * nu);
+  pc.mode = mode;
+  const double N = (double)pl.nnodes;
+  pc.scale = mode == 0 ? pl.cell_volume / N : 1.0 / (pl.cell_volume * N);
+  hipError_t e = hipSuccess;
+  auto mark = [&](int i) {
+    if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
+  };
+  mark(0);
+  if (pl.dtype == 0) {
+    if (e == hipSuccess)
+      e = launch_plane_fwd<float>((const float*)in, ws, pl.plane, pl.plane_blocks,
+                                  pl.plane_threads, pl.plane_lds, st);
+    mark(1);
+    if (e == hipSuccess) e = launch_pencil<float>(ws, pc, pl.pencil_threads, pl.pencil_lds, st);
+    mark(2);
+    if (e == hipSuccess)
+      e = launch_plane_inv<float>(ws, (float*)out, pl.plane, pl.plane_blocks, pl.plane_threads,
+                                  pl.plane_lds, st);
+  } else {
+    if (e == hipSuccess)
+      e = launch_plane_fwd<double>((const double*)in, ws, pl.plane, pl.plane_blocks,
+                                   pl.plane_threads, pl.plane_lds, st);
+    mark(1);
+    if (e == hipSuccess) e = launch_pencil<double>(ws, pc, pl.pencil_threads, pl.pencil_lds, st);
+    mark(2);
+    if (e == hipSuccess)
+      e = launch_plane_inv<double>(ws, (double*)out, pl.plane, pl.plane_blocks,
+                                   pl.plane_threads, pl.plane_lds, st);
+  }
+  mark(3);
+  if (e != hipSuccess) return fail(3, std::string("kernel launch: ") + hipGetErrorString(e));
+  return Status{};
+}
+
+}  // namespace b17

@@ -1,0 +1,73 @@
+// Host-side planning: radix factorisation, launch geometry, device-resident
+// per-axis twiddle and modal-coefficient tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace b17 {
+
+// thread-local last-error string of the C ABI
+void set_error(const std::string& msg);
+const std::string& get_error();
+
+struct Status {
+  int code = 0;
+  std::string msg;
+  bool ok() const { return code == 0; }
+};
+
+// Tunables of one plan (chosen by defaults or by the autotuner).
+struct Tuning {
+  int plane_split = 0;       // middle-axis split P (3D); 0 = automatic
+  int pencil_width = 0;      // pencil tile width W; 0 = automatic
+  size_t lds_budget = 160 * 1024;
+  int xcd_remap = 1;
+  std::vector<int> radices[3];  // forced radix orders per FFT length role
+};
+
+// Radix order (largest first) for a Stockham transform of length n, or empty
+// if n has a prime factor > 7.
+std::vector<int> default_radices(int n);
+bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
+                   std::string* why);
+
+struct Plan {
+  int dim = 0, dtype = 0;
+  int shape[3] = {1, 1, 1};
+  double L[3] = {1, 1, 1};
+  size_t esz = 0;  // sizeof(T)
+  int64_t nnodes = 0;
+  double cell_volume = 1.0;
+
+  // device tables (owned)
+  void* tw[3] = {nullptr, nullptr, nullptr};
+  void* coef[3] = {nullptr, nullptr, nullptr};
+
+  PlaneParams plane{};
+  int plane_blocks = 0, plane_threads = 0;
+  size_t plane_lds = 0;
+  PencilParams pencil{};
+  int pencil_threads = 0;
+  size_t pencil_lds = 0;
+  size_t ws_bytes = 0;
+  Tuning tuning;
+
+  ~Plan();
+};
+
+Status build_plan(int dim, const int* shape, const double* L, int dtype,
+                  const Tuning& tun, std::shared_ptr<Plan>& out);
+
+// Launch the three-kernel pipeline.  mode 0: K u, mode 1: K^-1 f.
+// `ev`, if not null, holds 4 events recorded before the first kernel and after
+// each of the three kernels (per-kernel timing).
+Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in,
+                    void* out, void* ws, hipStream_t st, hipEvent_t* ev = nullptr);
+
+}  // namespace b17

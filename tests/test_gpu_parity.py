@@ -1,0 +1,177 @@
+"""Parity of the HIP path (libbri17_amd.so through the C ABI) with the CPU
+oracle and with the reference's own known-answer tests.
+
+Tolerances: fp64 results within 1e-12 relative (2-norm) of the fp64 oracle;
+fp32 results within 1e-5 relative of the fp64 oracle applied to the same
+(fp32-rounded) input; the reference's dense-operator test at its own
+rtol 1e-15 / atol 1e-14."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import assemble_stiffness, assert_equal, rel_err
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+
+
+def _dev_run(device, op, x, mode="apply"):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(x)).to(device)
+    out = op.apply(t) if mode == "apply" else op.solve(t)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def _op(shape, L, mu, nu, dtype):
+    from bri17_amd import StiffnessMatrix
+    return StiffnessMatrix(shape, L, mu, nu, dtype)
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_reference_stiffness_section_on_device(device, oracle, fixtures, D):
+    """tests/test_bri17.cpp:337-361 / :363-536 re-run on the device apply:
+    probe K column by column (:130-150) and compare with the assembly of the
+    Maxima Ke (:153-174) at rtol 1e-15, atol 1e-14."""
+    import torch
+    shape, L = tuple(fixtures["shape%d" % D]), fixtures["L%d" % D]
+    op = _op(shape, L, fixtures["mu"], fixtures["nu"], "float64")
+    K_exp = assemble_stiffness(oracle, shape, L, fixtures["Ke%d" % D])
+    n = K_exp.shape[0]
+    u = torch.zeros((n,) + (D,) + shape, dtype=torch.float64, device=device)
+    u.view(n, n)[torch.arange(n), torch.arange(n)] = 1.0
+    out = torch.empty_like(u)
+    for j in range(n):
+        op.apply(u[j], out=out[j])
+    torch.cuda.synchronize()
+    K_act = out.view(n, n).cpu().numpy().T
+    assert_equal(K_exp, K_act, fixtures["rtol"], fixtures["atol"])
+
+
+CASES = [
+    ((64, 64), (1.0, 1.0)),            # 2D Q4 64x64 unit square
+    ((3, 4), (3.3, 4.8)),              # reference 2D fixture grid
+    ((3, 4, 5), (3.3, 4.8, 6.5)),      # reference 3D fixture grid (radix 3 and 5)
+    ((6, 8, 10), (1.0, 1.5, 2.0)),
+    ((16, 18, 20), (1.0, 1.0, 1.0)),
+    ((32, 32, 32), (1.0, 1.0, 1.0)),
+    ((27, 12, 10), (2.0, 1.0, 0.5)),
+    ((48, 36, 54), (1.0, 1.0, 1.0)),
+    ((81, 9, 16), (1.0, 1.0, 1.0)),
+    ((7, 14, 49), (1.0, 2.0, 3.0)),
+    ((5, 7), (1.0, 1.0)),
+    ((243, 12), (1.0, 1.0)),
+    ((512, 96), (1.0, 0.2)),
+    ((1, 6, 5), (1.0, 1.0, 1.0)),
+]
+
+
+def _field(oracle, seed, shape, dist="uniform"):
+    D = len(shape)
+    n = D * int(np.prod(shape))
+    if dist == "uniform":
+        return oracle.fill_uniform(seed, n).reshape((D,) + tuple(shape))
+    return oracle.fill_normal(seed, n).reshape((D,) + tuple(shape))
+
+
+@pytest.mark.parametrize("shape,L", CASES)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_apply_matches_oracle(device, oracle, shape, L, dtype):
+    mu, nu = 1.0, 0.3
+    u = _field(oracle, 1, shape).astype(dtype)
+    print("apply", shape, dtype, flush=True)
+    got = _dev_run(device, _op(shape, L, mu, nu, dtype), u)
+    ref = oracle.apply(shape, L, mu, nu, u.astype(np.float64), nthreads=NT)
+    err = rel_err(got, ref)
+    print("  rel err %.3e" % err)
+    assert err <= (1e-12 if dtype == "float64" else 1e-5)
+
+
+@pytest.mark.parametrize("shape,L", CASES)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_solve_matches_oracle(device, oracle, shape, L, dtype):
+    mu, nu = 0.7, 0.25
+    f = _field(oracle, 2, shape, "normal")
+    f -= f.mean(axis=tuple(range(1, len(shape) + 1)), keepdims=True)
+    f = f.astype(dtype)
+    print("solve", shape, dtype, flush=True)
+    got = _dev_run(device, _op(shape, L, mu, nu, dtype), f, "solve")
+    ref = oracle.solve(shape, L, mu, nu, f.astype(np.float64), nthreads=NT)
+    err = rel_err(got, ref)
+    print("  rel err %.3e" % err)
+    assert err <= (1e-12 if dtype == "float64" else 1e-5)
+
+
+def test_config1_2d_64_against_element_product(device, oracle):
+    """2D plane-strain Q4 on the periodic 64x64 unit square, fp64, mu=1,
+    nu=0.3: device K u against the reference apply restatement and against
+    the assembled element-stiffness product."""
+    shape, L, mu, nu = (64, 64), (1.0, 1.0), 1.0, 0.3
+    u = _field(oracle, 101, shape)
+    got = _dev_run(device, _op(shape, L, mu, nu, "float64"), u)
+    assert rel_err(got, oracle.apply(shape, L, mu, nu, u)) <= 1e-12
+    Ke = oracle.element_stiffness([1 / 64, 1 / 64], mu, nu)
+    assert rel_err(got, oracle.ebe_apply(shape, L, Ke, u)) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_config2_3d_128(device, oracle, dtype):
+    """3D H8, 128^3 unit cube, mu=1, nu=0.3, u ~ U[-1,1]: single apply against
+    the fp64 oracle and the element-by-element product."""
+    shape, L, mu, nu = (128, 128, 128), (1.0, 1.0, 1.0), 1.0, 0.3
+    u = _field(oracle, 202, shape).astype(dtype)
+    got = _dev_run(device, _op(shape, L, mu, nu, dtype), u)
+    u64 = u.astype(np.float64)
+    ref = oracle.apply(shape, L, mu, nu, u64, nthreads=NT)
+    err = rel_err(got, ref)
+    print("128^3 %s rel err vs oracle %.3e" % (dtype, err))
+    assert err <= (1e-12 if dtype == "float64" else 1e-5)
+    if dtype == "float64":
+        Ke = oracle.element_stiffness([1 / 128] * 3, mu, nu)
+        assert rel_err(got, oracle.ebe_apply(shape, L, Ke, u64, nthreads=NT)) <= 1e-12
+
+
+def test_config3_3d_256_fp32(device, oracle):
+    """3D H8 at 256^3, fp32 with the fp64 oracle, mu=1, nu=0.25, f ~ N(0,1)
+    with zero mean: K f and the Green solve K u = f, relative error of each."""
+    shape, L, mu, nu = (256, 256, 256), (1.0, 1.0, 1.0), 1.0, 0.25
+    f = _field(oracle, 303, shape, "normal")
+    f -= f.mean(axis=(1, 2, 3), keepdims=True)
+    f32 = f.astype(np.float32)
+    f64 = f32.astype(np.float64)
+    op = _op(shape, L, mu, nu, "float32")
+    ku = _dev_run(device, op, f32)
+    e1 = rel_err(ku, oracle.apply(shape, L, mu, nu, f64, nthreads=NT))
+    gu = _dev_run(device, op, f32, "solve")
+    e2 = rel_err(gu, oracle.solve(shape, L, mu, nu, f64, nthreads=NT))
+    print("256^3 fp32 rel err: K.u %.3e  Green %.3e" % (e1, e2))
+    assert e1 <= 1e-5 and e2 <= 1e-5
+
+
+def test_deterministic_and_linear(device, oracle):
+    import torch
+    shape = (48, 36, 54)
+    op = _op(shape, (1.0, 1.0, 1.0), 1.2, 0.3, "float32")
+    u = torch.from_numpy(_field(oracle, 7, shape).astype(np.float32)).to(device)
+    v = torch.from_numpy(_field(oracle, 8, shape).astype(np.float32)).to(device)
+    a = op.apply(u)
+    b = op.apply(u)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    lin = op.apply(u + 2 * v) - a - 2 * op.apply(v)
+    assert float(lin.norm() / a.norm()) < 1e-5
+
+
+def test_round_trip_green_full_size(device, oracle):
+    """Size-independent property at 256^3 fp32: K G f = f - mean(f)."""
+    import torch
+    shape = (256, 256, 256)
+    op = _op(shape, (1.0, 1.0, 1.0), 1.0, 0.25, "float32")
+    g = torch.Generator(device=device).manual_seed(5)
+    f = torch.randn((3,) + shape, device=device, dtype=torch.float32, generator=g)
+    f -= f.mean(dim=(1, 2, 3), keepdim=True)
+    kgf = op.apply(op.solve(f))
+    torch.cuda.synchronize()
+    assert float((kgf - f).norm() / f.norm()) < 1e-5
