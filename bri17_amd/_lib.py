@@ -46,9 +46,32 @@ SIGNATURES = {
     "bri17_operator_solve": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "bri17_operator_apply_host": (_c.c_int, [_vp, _vp, _vp]),
     "bri17_operator_solve_host": (_c.c_int, [_vp, _vp, _vp]),
+    "bri17_operator_create_tuned": (
+        _c.c_int, [_c.c_int, _ip, _dp, _c.c_double, _c.c_double, _c.c_int, _vp, _c.POINTER(_vp)]),
+    "bri17_operator_describe": (_c.c_int, [_vp, _c.c_char_p, _c.c_size_t]),
     "bri17_operator_profile": (_c.c_int, [_vp, _c.c_int]),
     "bri17_operator_profile_read": (_c.c_int, [_vp, _ip, _dp]),
 }
+
+
+
+class Tuning(ctypes.Structure):
+    """bri17_tuning of include/bri17_amd.h."""
+    _fields_ = [("plane_split", ctypes.c_int), ("pencil_width", ctypes.c_int),
+                ("lds_budget_kb", ctypes.c_int), ("xcd_remap", ctypes.c_int),
+                ("radices", (ctypes.c_int * 8) * 3), ("force_generic", ctypes.c_int)]
+
+    @classmethod
+    def make(cls, plane_split=0, pencil_width=0, lds_budget_kb=0, xcd_remap=-1, radices=(None, None, None),
+             force_generic=0):
+        t = cls()
+        t.force_generic = force_generic
+        t.plane_split, t.pencil_width, t.lds_budget_kb, t.xcd_remap = plane_split, pencil_width, lds_budget_kb, xcd_remap
+        for r, rs in enumerate(radices):
+            for i, v in enumerate(rs or ()):
+                t.radices[r][i] = v
+        return t
+
 
 _lib = None
 

@@ -327,10 +327,27 @@ int bri17_modal_eigenstress_to_opposite_strain_f32(int dim, const int* shape, co
 
 int bri17_operator_create(int dim, const int* shape, const double* L, double mu, double nu,
                           int dtype, bri17_operator** out) {
+  return bri17_operator_create_tuned(dim, shape, L, mu, nu, dtype, nullptr, out);
+}
+
+int bri17_operator_create_tuned(int dim, const int* shape, const double* L, double mu,
+                                double nu, int dtype, const bri17_tuning* tuning,
+                                bri17_operator** out) {
   if (!out || !shape || !L) return err(BRI17_ERR_INVALID, "null argument");
   *out = nullptr;
+  b17::Tuning tun;
+  if (tuning) {
+    tun.plane_split = tuning->plane_split;
+    tun.pencil_width = tuning->pencil_width;
+    if (tuning->lds_budget_kb > 0) tun.lds_budget = (size_t)tuning->lds_budget_kb * 1024;
+    if (tuning->xcd_remap >= 0) tun.xcd_remap = tuning->xcd_remap;
+    tun.force_generic = tuning->force_generic;
+    for (int r = 0; r < 3; r++)
+      for (int i = 0; i < 8 && tuning->radices[r][i] > 0; i++)
+        tun.radices[r].push_back(tuning->radices[r][i]);
+  }
   std::shared_ptr<Plan> pl;
-  int rc = finish(b17::build_plan(dim, shape, L, dtype, b17::Tuning{}, pl));
+  int rc = finish(b17::build_plan(dim, shape, L, dtype, tun, pl));
   if (rc) return rc;
   auto* op = new (std::nothrow) bri17_operator;
   if (!op) return err(BRI17_ERR_NOMEM, "out of host memory");
@@ -338,6 +355,15 @@ int bri17_operator_create(int dim, const int* shape, const double* L, double mu,
   op->mu = mu;
   op->nu = nu;
   *out = op;
+  return BRI17_OK;
+}
+
+int bri17_operator_describe(const bri17_operator* op, char* buf, size_t len) {
+  if (!op || !buf || len == 0) return err(BRI17_ERR_INVALID, "null argument");
+  std::string s = b17::describe_plan(*op->plan);
+  size_t n = std::min(len - 1, s.size());
+  std::memcpy(buf, s.data(), n);
+  buf[n] = '\0';
   return BRI17_OK;
 }
 

@@ -10,10 +10,34 @@
 // spectrum along the last axis is formed (R2C / C2R), which is exact.
 #include "fft_core.hpp"
 #include "kernels.hpp"
+#include "modal.hpp"
 
 namespace b17 {
 
 constexpr int kMaxE = 16;  // max complex elements per thread per FFT pass
+// global loads a thread keeps in flight when staging (fewer for fp64: VGPRs)
+template <typename T>
+constexpr int kLoads = sizeof(T) == 4 ? 16 : 8;
+
+// Copy `tot` elements with up to LB loads in flight per thread: all loads of
+// a batch are issued before any of their values is consumed, so a staging
+// loop is bound by bandwidth instead of one memory latency per element.
+template <int LB, typename V, typename Get, typename Put>
+__device__ __forceinline__ void staged_copy(int tot, int tid, int nthr, Get get, Put put) {
+  for (int e0 = tid; e0 < tot; e0 += LB * nthr) {
+    V v[LB];
+#pragma unroll
+    for (int i = 0; i < LB; i++) {
+      const int e = e0 + i * nthr;
+      if (e < tot) v[i] = get(e);
+    }
+#pragma unroll
+    for (int i = 0; i < LB; i++) {
+      const int e = e0 + i * nthr;
+      if (e < tot) put(e, v[i]);
+    }
+  }
+}
 
 template <typename T>
 __device__ __forceinline__ cpx<T> post_r2c(cpx<T> A, cpx<T> B, cpx<T> w) {
@@ -54,21 +78,22 @@ __global__ void __launch_bounds__(1024)
   const int pst = pp.p;
 
   // 1. rows -> LDS (even rows are read as m complex pairs)
-  if (pp.even) {
-    const int m = pp.m, tot = nrows * m;
-    for (int e = tid; e < tot; e += nthr) {
+  {
+    const int m = pp.m, tot = nrows * m, rsl = pp.rsl;
+    auto put = [&](int e, cpx<T> v) {
       const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      const int mm = e - qq * m;
-      const cpx<T>* src =
-          reinterpret_cast<const cpx<T>*>(u + (row0 + (int64_t)qq * pst) * pp.nl);
-      lds[qq * pp.rsl + mm] = src[mm];
-    }
-  } else {
-    const int m = pp.m, tot = nrows * m;
-    for (int e = tid; e < tot; e += nthr) {
-      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      const int mm = e - qq * m;
-      lds[qq * pp.rsl + mm] = {u[(row0 + (int64_t)qq * pst) * pp.nl + mm], T(0)};
+      lds[qq * rsl + (e - qq * m)] = v;
+    };
+    if (pp.even) {
+      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
+        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+        return reinterpret_cast<const cpx<T>*>(u + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m];
+      }, put);
+    } else {
+      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
+        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+        return cpx<T>{u[(row0 + (int64_t)qq * pst) * pp.nl + (e - qq * m)], T(0)};
+      }, put);
     }
   }
   __syncthreads();
@@ -120,14 +145,17 @@ __global__ void __launch_bounds__(1024)
 
   // 5. store
   {
-    const int tot = nrows * pp.nlh;
-    for (int e = tid; e < tot; e += nthr) {
+    const int tot = nrows * pp.nlh, nlh = pp.nlh;
+    cpx<T>* dst = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
+    staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
       const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      const int k = e - ka * pp.nlh;
-      cpx<T> v = lds[ka * pp.rsl + k];
+      cpx<T> v = lds[ka * pp.rsl + (e - ka * nlh)];
       if (p > 0) v = cmul(v, twc[p * ka]);
-      ws[((int64_t)plane * pp.n1 + p * pp.q + ka) * pp.rs + k] = v;
-    }
+      return v;
+    }, [&](int e, cpx<T> v) {
+      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
+      dst[(int64_t)ka * pp.rs + (e - ka * nlh)] = v;
+    });
   }
 }
 
@@ -153,14 +181,16 @@ __global__ void __launch_bounds__(1024)
 
   // 1. load (undo the split twiddle)
   {
-    const int tot = nrows * pp.nlh;
-    for (int e = tid; e < tot; e += nthr) {
+    const int tot = nrows * pp.nlh, nlh = pp.nlh;
+    const cpx<T>* src = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
+    staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
       const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      const int k = e - ka * pp.nlh;
-      cpx<T> v = ws[((int64_t)plane * pp.n1 + p * pp.q + ka) * pp.rs + k];
+      return src[(int64_t)ka * pp.rs + (e - ka * nlh)];
+    }, [&](int e, cpx<T> v) {
+      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
       if (p > 0) v = cmulc(v, twc[p * ka]);
-      lds[ka * pp.rsl + k] = v;
-    }
+      lds[ka * pp.rsl + (e - ka * nlh)] = v;
+    });
   }
   __syncthreads();
 
@@ -220,16 +250,21 @@ __global__ void __launch_bounds__(1024)
 
   // 5. store real rows
   {
-    const int m = pp.m, tot = nrows * m;
-    for (int e = tid; e < tot; e += nthr) {
+    const int m = pp.m, tot = nrows * m, rsl = pp.rsl;
+    auto get = [&](int e) {
       const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      const int mm = e - qq * m;
-      if (pp.even) {
-        cpx<T>* dst = reinterpret_cast<cpx<T>*>(out + (row0 + (int64_t)qq * pst) * pp.nl);
-        dst[mm] = lds[qq * pp.rsl + mm];
-      } else {
-        out[(row0 + (int64_t)qq * pst) * pp.nl + mm] = lds[qq * pp.rsl + mm].x;
-      }
+      return lds[qq * rsl + (e - qq * m)];
+    };
+    if (pp.even) {
+      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, get, [&](int e, cpx<T> v) {
+        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+        reinterpret_cast<cpx<T>*>(out + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m] = v;
+      });
+    } else {
+      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, get, [&](int e, cpx<T> v) {
+        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+        out[(row0 + (int64_t)qq * pst) * pp.nl + (e - qq * m)] = v.x;
+      });
     }
   }
 }
@@ -270,75 +305,6 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
   __syncthreads();
 }
 
-// The modal DIM x DIM block at frequency k (bri17.hpp:255-288), applied to x
-// (MODE 0: y = scale K^ x) or solved against x (MODE 1: y = scale K^-1 x,
-// y = 0 at k = 0).  K^ is real symmetric.
-template <typename T, int DIM, int MODE>
-__device__ __forceinline__ void modal_block(const ModalCoef<T>* const* coef, int k0,
-                                            int k1, int kl, T mu, T s, T scale,
-                                            cpx<T>* x) {
-  if constexpr (DIM == 2) {
-    const ModalCoef<T> a = coef[0][k0], b = coef[1][kl];
-    const T H0 = a.phi * b.chi, H1 = a.chi * b.phi;
-    const T dg = mu * (H0 + H1);
-    const T K00 = s * H0 + dg, K11 = s * H1 + dg, K01 = s * a.psi * b.psi;
-    if constexpr (MODE == 0) {
-      const cpx<T> y0 = (x[0] * K00 + x[1] * K01) * scale;
-      const cpx<T> y1 = (x[0] * K01 + x[1] * K11) * scale;
-      x[0] = y0;
-      x[1] = y1;
-    } else {
-      if (k0 == 0 && kl == 0) {
-        x[0] = {T(0), T(0)};
-        x[1] = {T(0), T(0)};
-        return;
-      }
-      const T r = scale / (K00 * K11 - K01 * K01);
-      const cpx<T> y0 = (x[0] * K11 - x[1] * K01) * r;
-      const cpx<T> y1 = (x[1] * K00 - x[0] * K01) * r;
-      x[0] = y0;
-      x[1] = y1;
-    }
-  } else {
-    const ModalCoef<T> a = coef[0][k0], b = coef[1][k1], c = coef[2][kl];
-    const T H0 = a.phi * b.chi * c.chi;
-    const T H1 = a.chi * b.phi * c.chi;
-    const T H2 = a.chi * b.chi * c.phi;
-    const T dg = mu * (H0 + H1 + H2);
-    const T K00 = s * H0 + dg, K11 = s * H1 + dg, K22 = s * H2 + dg;
-    const T K01 = s * a.psi * b.psi * c.chi;
-    const T K02 = s * a.psi * b.chi * c.psi;
-    const T K12 = s * a.chi * b.psi * c.psi;
-    if constexpr (MODE == 0) {
-      const cpx<T> y0 = (x[0] * K00 + x[1] * K01 + x[2] * K02) * scale;
-      const cpx<T> y1 = (x[0] * K01 + x[1] * K11 + x[2] * K12) * scale;
-      const cpx<T> y2 = (x[0] * K02 + x[1] * K12 + x[2] * K22) * scale;
-      x[0] = y0;
-      x[1] = y1;
-      x[2] = y2;
-    } else {
-      if (k0 == 0 && k1 == 0 && kl == 0) {
-        x[0] = x[1] = x[2] = {T(0), T(0)};
-        return;
-      }
-      // adjugate of the symmetric SPD block
-      const T A00 = K11 * K22 - K12 * K12;
-      const T A01 = K02 * K12 - K01 * K22;
-      const T A02 = K01 * K12 - K02 * K11;
-      const T A11 = K00 * K22 - K02 * K02;
-      const T A12 = K01 * K02 - K00 * K12;
-      const T A22 = K00 * K11 - K01 * K01;
-      const T r = scale / (K00 * A00 + K01 * A01 + K02 * A02);
-      const cpx<T> y0 = (x[0] * A00 + x[1] * A01 + x[2] * A02) * r;
-      const cpx<T> y1 = (x[0] * A01 + x[1] * A11 + x[2] * A12) * r;
-      const cpx<T> y2 = (x[0] * A02 + x[1] * A12 + x[2] * A22) * r;
-      x[0] = y0;
-      x[1] = y1;
-      x[2] = y2;
-    }
-  }
-}
-
 template <typename T, int DIM, int MODE>
 __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const PencilParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -363,13 +329,13 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
   const int cst = N0 * PW;  // LDS distance between components
 
   // load the tile: element (c, i0, p, w) at ((c N0 + i0) P + p) W + w
-  for (int e = tid; e < tot; e += nthr) {
+  staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
     const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
     const int w = e - rest * W;
     cpx<T> v = {T(0), T(0)};
     if (w < wv) v = ws[(int64_t)rest * pp.f + f0 + w];
-    lds[e] = v;
-  }
+    return v;
+  }, [&](int e, cpx<T> v) { lds[e] = v; });
   __syncthreads();
   if (P > 1) pdft<T, false>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
 
@@ -418,11 +384,11 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
   }
   if (P > 1) pdft<T, true>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
 
-  for (int e = tid; e < tot; e += nthr) {
+  staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) { return lds[e]; }, [&](int e, cpx<T> v) {
     const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
     const int w = e - rest * W;
-    if (w < wv) ws[(int64_t)rest * pp.f + f0 + w] = lds[e];
-  }
+    if (w < wv) ws[(int64_t)rest * pp.f + f0 + w] = v;
+  });
 }
 
 // ------------------------------------------------------------------ launchers

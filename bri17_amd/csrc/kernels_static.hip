@@ -1,0 +1,550 @@
+// Compile-time-size fast path of the periodic stiffness operator (3D grids
+// whose sides factor as two radix passes, e.g. 64, 128, 256).
+//
+// Same pipeline and workspace layout as kernels.hip (plane R2C + first
+// middle-axis stage -> pencil: last middle-axis stage, axis-0 FFT, modal
+// block, inverse -> plane inverse + C2R), but every transform is a two-pass
+// Stockham FFT held in registers: the first pass reads global memory straight
+// into VGPRs, one LDS exchange feeds the second pass, and the row -> column
+// transposition is the only other trip through LDS.  With all sizes known at
+// compile time the index arithmetic folds to shifts and constants.
+#include "fft_core.hpp"
+#include "kernels.hpp"
+#include "kernels_static.hpp"
+#include "modal.hpp"
+
+namespace b17 {
+
+// one pad element every 32 complex entries: breaks the power-of-two strides
+// of the exchanges across LDS banks
+__device__ __forceinline__ int pd(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int pd_size(int n) { return n + (n >> 5) + 1; }
+
+template <typename T>
+__device__ __forceinline__ cpx<T> s_post_r2c(cpx<T> A, cpx<T> B, cpx<T> w) {
+  // X[k] = (A + conj B)/2 - i w (A - conj B)/2 with A = Z[k], B = Z[M-k]
+  const cpx<T> e = {T(0.5) * (A.x + B.x), T(0.5) * (A.y - B.y)};
+  const cpx<T> d = {T(0.5) * (A.x - B.x), T(0.5) * (A.y + B.y)};
+  return e + cmul(w, cpx<T>{d.y, -d.x});
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> s_pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
+  // Z[k] = (A + conj B) + i conj(w) (A - conj B) with A = X[k], B = X[M-k]
+  const cpx<T> e = {A.x + B.x, A.y - B.y};
+  const cpx<T> o = cmulc(cpx<T>{A.x - B.x, A.y + B.y}, w);
+  return {e.x - o.y, e.y + o.x};
+}
+
+template <typename T>
+__device__ __forceinline__ void copy_table(cpx<T>* dst, const cpx<T>* src, int n, int tid,
+                                           int nthr) {
+  for (int i = tid; i < n; i += nthr) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------------ plane fwd
+//
+// Workgroup (c, i0, p): rows i1 = q P + p, q < Q, of M complex (= 2M reals).
+// Row FFT M = R1 x R2 by M/R1 threads per row; column stage Q = C1 x C2 by
+// Q/C1 threads per column on M column sequences (columns 0 and M, both real
+// after the row transform, are packed into one complex column).
+
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
+__global__ void __launch_bounds__(Q * M / R1)
+    k_plane_fwd_s(const T* __restrict__ u, cpx<T>* __restrict__ ws, const cpx<T>* __restrict__ g_twr,
+                  const cpx<T>* __restrict__ g_twc, int rs) {
+  constexpr int NT = Q * M / R1;
+  constexpr int TPR = M / R1;
+  constexpr int TPC = Q / C1;
+  constexpr int N1 = P * Q;
+  constexpr int RS = M;  // row stride (before padding)
+  static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
+  static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* twr = buf + pd_size(Q * RS);   // w_{2M}^j, j < 2M
+  cpx<T>* twc = twr + 2 * M;             // w_{N1}^j, j < N1
+  cpx<T>* scr = twc + N1;                // packed-column scratch, Q entries
+  const int tid = threadIdx.x;
+  const int plane = blockIdx.x / P;
+  const int p = blockIdx.x - plane * P;
+
+  copy_table(twr, g_twr, 2 * M, tid, NT);
+  copy_table(twc, g_twc, N1, tid, NT);
+
+  // ---- rows: pass 1 straight from global memory
+  {
+    const int q = tid / TPR, j = tid - (tid / TPR) * TPR;
+    const cpx<T>* src = reinterpret_cast<const cpx<T>*>(u + ((int64_t)plane * N1 + q * P + p) * (2 * M));
+    cpx<T> v[R1];
+#pragma unroll
+    for (int k = 0; k < R1; k++) v[k] = src[j + k * TPR];
+    bfly<R1, false>(v);
+#pragma unroll
+    for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
+    __syncthreads();
+    // pass 2: R1/R2 butterflies per thread, natural-order output
+    cpx<T> w[R1 / R2][R2];
+#pragma unroll
+    for (int b = 0; b < R1 / R2; b++) {
+      const int jj = j + b * TPR;
+#pragma unroll
+      for (int k = 0; k < R2; k++) {
+        w[b][k] = buf[pd(q * RS + jj + k * R1)];
+        if (k) w[b][k] = cmul(w[b][k], twr[2 * jj * k]);
+      }
+      bfly<R2, false>(w[b]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < R1 / R2; b++)
+#pragma unroll
+      for (int k = 0; k < R2; k++) buf[pd(q * RS + j + b * TPR + k * R1)] = w[b][k];
+    __syncthreads();
+  }
+
+  // ---- columns: pass 1 reads the split spectrum of two rows' worth of Z
+  const int cs = tid % M, l = tid / M;
+  cpx<T> x[C1];
+#pragma unroll
+  for (int kk = 0; kk < C1; kk++) {
+    const int qq = l + kk * TPC;
+    const cpx<T> A = buf[pd(qq * RS + cs)];
+    if (cs == 0) {
+      // X[q][0] = Re Z + Im Z, X[q][M] = Re Z - Im Z, packed as X0 + i XM
+      x[kk] = {A.x + A.y, A.x - A.y};
+    } else {
+      x[kk] = s_post_r2c(A, buf[pd(qq * RS + M - cs)], twr[cs]);
+    }
+  }
+  bfly<C1, false>(x);
+  __syncthreads();
+  // column exchange layout: element idx of column cs at pd(idx * M + cs)
+#pragma unroll
+  for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
+  __syncthreads();
+  cpx<T> y[C1 / C2][C2];
+#pragma unroll
+  for (int b = 0; b < C1 / C2; b++) {
+    const int jj = l + b * TPC;
+#pragma unroll
+    for (int k = 0; k < C2; k++) {
+      y[b][k] = buf[pd((jj + k * C1) * M + cs)];
+      if (k) y[b][k] = cmul(y[b][k], twc[P * jj * k]);
+    }
+    bfly<C2, false>(y[b]);
+  }
+  // packed column: Y0[ka] = (Z[ka] + conj Z[-ka])/2, YM[ka] = -i (Z[ka] - conj Z[-ka])/2
+  if (cs == 0) {
+#pragma unroll
+    for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+      for (int k = 0; k < C2; k++) scr[l + b * TPC + k * C1] = y[b][k];
+  }
+  __syncthreads();
+  cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * rs;
+#pragma unroll
+  for (int b = 0; b < C1 / C2; b++) {
+#pragma unroll
+    for (int k = 0; k < C2; k++) {
+      const int ka = l + b * TPC + k * C1;
+      cpx<T> v = y[b][k];
+      if (cs == 0) {
+        const cpx<T> zn = scr[(Q - ka) % Q];
+        cpx<T> y0 = {T(0.5) * (v.x + zn.x), T(0.5) * (v.y - zn.y)};
+        cpx<T> ym = {T(0.5) * (v.y + zn.y), T(0.5) * (zn.x - v.x)};
+        if (P > 1 && p) {
+          const cpx<T> tw = twc[p * ka];
+          y0 = cmul(y0, tw);
+          ym = cmul(ym, tw);
+        }
+        dst[(int64_t)ka * rs] = y0;
+        dst[(int64_t)ka * rs + M] = ym;
+      } else {
+        if (P > 1 && p) v = cmul(v, twc[p * ka]);
+        dst[(int64_t)ka * rs + cs] = v;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ plane inv
+
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
+__global__ void __launch_bounds__(Q * M / R1)
+    k_plane_inv_s(const cpx<T>* __restrict__ ws, T* __restrict__ out,
+                  const cpx<T>* __restrict__ g_twr, const cpx<T>* __restrict__ g_twc, int rs) {
+  constexpr int NT = Q * M / R1;
+  constexpr int TPR = M / R1;
+  constexpr int TPC = Q / C1;
+  constexpr int N1 = P * Q;
+  constexpr int RS = M + 1;  // rows keep X[q][M] for the C2R split
+  static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
+  static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* twr = buf + pd_size(Q * RS > Q * M ? Q * RS : Q * M);
+  cpx<T>* twc = twr + 2 * M;
+  const int tid = threadIdx.x;
+  const int plane = blockIdx.x / P;
+  const int p = blockIdx.x - plane * P;
+
+  copy_table(twr, g_twr, 2 * M, tid, NT);
+  copy_table(twc, g_twc, N1, tid, NT);
+
+  // ---- columns: inverse Q-point stage, pass 1 straight from global memory
+  const int cs = tid % M, l = tid / M;
+  const cpx<T>* src = ws + ((int64_t)plane * N1 + p * Q) * rs;
+  cpx<T> x[C1];
+#pragma unroll
+  for (int kk = 0; kk < C1; kk++) {
+    const int ka = l + kk * TPC;
+    cpx<T> v = src[(int64_t)ka * rs + cs];
+    if (cs == 0) {
+      // columns 0 and M are both Hermitian sequences: pack as X0 + i XM
+      const cpx<T> vm = src[(int64_t)ka * rs + M];
+      v = {v.x - vm.y, v.y + vm.x};
+    }
+    x[kk] = v;
+  }
+  __syncthreads();  // twiddle tables ready
+  if (P > 1 && p) {
+#pragma unroll
+    for (int kk = 0; kk < C1; kk++) x[kk] = cmulc(x[kk], twc[p * (l + kk * TPC)]);
+  }
+  bfly<C1, true>(x);
+#pragma unroll
+  for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
+  __syncthreads();
+  cpx<T> y[C1 / C2][C2];
+#pragma unroll
+  for (int b = 0; b < C1 / C2; b++) {
+    const int jj = l + b * TPC;
+#pragma unroll
+    for (int k = 0; k < C2; k++) {
+      y[b][k] = buf[pd((jj + k * C1) * M + cs)];
+      if (k) y[b][k] = cmulc(y[b][k], twc[P * jj * k]);
+    }
+    bfly<C2, true>(y[b]);
+  }
+  __syncthreads();
+  // rows q = jj + C1 k, natural order; unpack columns 0 and M
+#pragma unroll
+  for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+    for (int k = 0; k < C2; k++) {
+      const int qq = l + b * TPC + k * C1;
+      const cpx<T> v = y[b][k];
+      if (cs == 0) {
+        buf[pd(qq * RS)] = {v.x, T(0)};
+        buf[pd(qq * RS + M)] = {v.y, T(0)};
+      } else {
+        buf[pd(qq * RS + cs)] = v;
+      }
+    }
+  __syncthreads();
+
+  // ---- rows: C2R split, then the inverse M-point transform
+  const int q = tid / TPR, j = tid - (tid / TPR) * TPR;
+  cpx<T> v[R1];
+#pragma unroll
+  for (int k = 0; k < R1; k++) {
+    const int mm = j + k * TPR;
+    const cpx<T> A = buf[pd(q * RS + mm)];
+    const cpx<T> B = buf[pd(q * RS + M - mm)];
+    if (mm == 0) {
+      // Z[0] = (X0 + conj XM) + i (X0 - conj XM) with X0, XM real
+      v[k] = {A.x + B.x, A.x - B.x};
+    } else {
+      v[k] = s_pre_c2r(A, B, twr[mm]);
+    }
+  }
+  bfly<R1, true>(v);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
+  __syncthreads();
+  cpx<T> w[R1 / R2][R2];
+#pragma unroll
+  for (int b = 0; b < R1 / R2; b++) {
+    const int jj = j + b * TPR;
+#pragma unroll
+    for (int k = 0; k < R2; k++) {
+      w[b][k] = buf[pd(q * RS + jj + k * R1)];
+      if (k) w[b][k] = cmulc(w[b][k], twr[2 * jj * k]);
+    }
+    bfly<R2, true>(w[b]);
+  }
+  cpx<T>* dst = reinterpret_cast<cpx<T>*>(out + ((int64_t)plane * N1 + q * P + p) * (2 * M));
+#pragma unroll
+  for (int b = 0; b < R1 / R2; b++)
+#pragma unroll
+    for (int k = 0; k < R2; k++) dst[j + b * TPR + k * R1] = w[b][k];
+}
+
+// ---------------------------------------------------------------------- pencil
+//
+// Tile (ka, f0..f0+W): all components c, all i0, all split indices p.  The
+// thread (c, w, l) owns i0 = l + k TPA (k < A0) of sequence (c, w) for every
+// p: it performs the split DFT over p and the first axis-0 pass in registers.
+
+template <typename T, int N0, int A0, int B0, int P, int W, int MODE>
+__global__ void __launch_bounds__(3 * W * (N0 / A0))
+    k_pencil_s(cpx<T>* __restrict__ ws, const PencilParams pp) {
+  constexpr int TPA = N0 / A0;
+  constexpr int NT = 3 * W * TPA;
+  constexpr int ROWS = 3 * P * N0;  // LDS rows of W entries: ((c P + kb) N0 + n0)
+  static_assert(A0 * B0 == N0 && A0 % B0 == 0, "pencil geometry");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* tw = buf + pd_size(ROWS * W);
+  const ModalCoef<T>* coef0 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[0]);
+  const ModalCoef<T>* coef1 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]);
+  const ModalCoef<T>* coef2 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[2]);
+  const int tid = threadIdx.x;
+  int t = blockIdx.x;
+  if (pp.xcd_remap) {
+    const int n = gridDim.x, per = n / 8, rem = n % 8;
+    const int xx = t % 8, idx = t / 8;
+    t = xx * per + (xx < rem ? xx : rem) + idx;
+  }
+  const int F = pp.f;
+  const int f0 = t * W;
+  const int wv = F - f0 < W ? F - f0 : W;
+  const int w = tid % W;
+  const int l = (tid / W) % TPA;
+  const int c = tid / (W * TPA);
+  const bool wok = w < wv;
+
+  copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, NT);
+
+  // load + split DFT over p + pass 1
+  cpx<T> v[P][A0];
+  {
+    const cpx<T>* src = ws + (int64_t)c * N0 * P * F + f0 + w;
+#pragma unroll
+    for (int k = 0; k < A0; k++)
+#pragma unroll
+      for (int pp_ = 0; pp_ < P; pp_++) {
+        const int i0 = l + k * TPA;
+        v[pp_][k] = wok ? src[((int64_t)i0 * P + pp_) * F] : cpx<T>{T(0), T(0)};
+      }
+  }
+  if constexpr (P > 1) {
+#pragma unroll
+    for (int k = 0; k < A0; k++) {
+      cpx<T> col[P];
+#pragma unroll
+      for (int pp_ = 0; pp_ < P; pp_++) col[pp_] = v[pp_][k];
+      bfly<P, false>(col);
+#pragma unroll
+      for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = col[pp_];
+    }
+  }
+#pragma unroll
+  for (int kb = 0; kb < P; kb++) {
+    bfly<A0, false>(v[kb]);
+#pragma unroll
+    for (int k = 0; k < A0; k++) buf[pd(((c * P + kb) * N0 + l * A0 + k) * W + w)] = v[kb][k];
+  }
+  __syncthreads();
+  // pass 2 -> natural order k0 = jj + A0 k
+  {
+    cpx<T> y[P][A0 / B0][B0];
+#pragma unroll
+    for (int kb = 0; kb < P; kb++)
+#pragma unroll
+      for (int b = 0; b < A0 / B0; b++) {
+        const int jj = l + b * TPA;
+#pragma unroll
+        for (int k = 0; k < B0; k++) {
+          y[kb][b][k] = buf[pd(((c * P + kb) * N0 + jj + k * A0) * W + w)];
+          if (k) y[kb][b][k] = cmul(y[kb][b][k], tw[jj * k]);
+        }
+        bfly<B0, false>(y[kb][b]);
+      }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < P; kb++)
+#pragma unroll
+      for (int b = 0; b < A0 / B0; b++)
+#pragma unroll
+        for (int k = 0; k < B0; k++)
+          buf[pd(((c * P + kb) * N0 + l + b * TPA + k * A0) * W + w)] = y[kb][b][k];
+  }
+  __syncthreads();
+
+  // modal block on every point (k0, kb, w) of the tile
+  {
+    const T mu = T(pp.mu), sc = T(pp.s), scale = T(pp.scale);
+    constexpr int NPTS = N0 * P * W;
+    for (int e = tid; e < NPTS; e += NT) {
+      const int ww = e % W;
+      const int kb = (e / W) % P;
+      const int k0 = e / (W * P);
+      if (ww >= wv) continue;
+      const int f = f0 + ww;
+      const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
+      const int kl = f - ka * pp.rs;
+      if (kl >= pp.nlh) continue;
+      const int k1 = ka + pp.q * kb;
+      const ModalCoef<T>* cf[3] = {coef0, coef1, coef2};
+      cpx<T> xx[3];
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) xx[cc] = buf[pd(((cc * P + kb) * N0 + k0) * W + ww)];
+      modal_block<T, 3, MODE>(cf, k0, k1, kl, mu, sc, scale, xx);
+#pragma unroll
+      for (int cc = 0; cc < 3; cc++) buf[pd(((cc * P + kb) * N0 + k0) * W + ww)] = xx[cc];
+    }
+  }
+  __syncthreads();
+
+  // inverse pass 1 (inputs k0 = l + k TPA)
+#pragma unroll
+  for (int kb = 0; kb < P; kb++) {
+#pragma unroll
+    for (int k = 0; k < A0; k++) v[kb][k] = buf[pd(((c * P + kb) * N0 + l + k * TPA) * W + w)];
+    bfly<A0, true>(v[kb]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int kb = 0; kb < P; kb++)
+#pragma unroll
+    for (int k = 0; k < A0; k++) buf[pd(((c * P + kb) * N0 + l * A0 + k) * W + w)] = v[kb][k];
+  __syncthreads();
+  // inverse pass 2 -> i0 = jj + A0 k, then the inverse split DFT and store
+  {
+    cpx<T> y[P][A0 / B0][B0];
+#pragma unroll
+    for (int kb = 0; kb < P; kb++)
+#pragma unroll
+      for (int b = 0; b < A0 / B0; b++) {
+        const int jj = l + b * TPA;
+#pragma unroll
+        for (int k = 0; k < B0; k++) {
+          y[kb][b][k] = buf[pd(((c * P + kb) * N0 + jj + k * A0) * W + w)];
+          if (k) y[kb][b][k] = cmulc(y[kb][b][k], tw[jj * k]);
+        }
+        bfly<B0, true>(y[kb][b]);
+      }
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int b = 0; b < A0 / B0; b++)
+#pragma unroll
+        for (int k = 0; k < B0; k++) {
+          cpx<T> col[P];
+#pragma unroll
+          for (int kb = 0; kb < P; kb++) col[kb] = y[kb][b][k];
+          bfly<P, true>(col);
+#pragma unroll
+          for (int kb = 0; kb < P; kb++) y[kb][b][k] = col[kb];
+        }
+    }
+    if (wok) {
+      cpx<T>* dst = ws + (int64_t)c * N0 * P * F + f0 + w;
+#pragma unroll
+      for (int b = 0; b < A0 / B0; b++)
+#pragma unroll
+        for (int k = 0; k < B0; k++) {
+          const int i0 = l + b * TPA + k * A0;
+#pragma unroll
+          for (int pp_ = 0; pp_ < P; pp_++) dst[((int64_t)i0 * P + pp_) * F] = y[pp_][b][k];
+        }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+
+template <typename K>
+static hipError_t set_lds_s(K kernel, size_t lds) {
+  if (lds > 65536)
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  return hipSuccess;
+}
+
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
+struct PlaneS {
+  static size_t lds_fwd() { return (pd_size(Q * M) + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
+  static size_t lds_inv() { return (pd_size(Q * (M + 1)) + 2 * M + P * Q) * sizeof(cpx<T>); }
+  static constexpr int threads() { return Q * M / R1; }
+};
+
+template <typename T, int N0, int A0, int B0, int P, int W>
+struct PencilS {
+  static size_t lds() { return (pd_size(3 * P * N0 * W) + N0) * sizeof(cpx<T>); }
+  static constexpr int threads() { return 3 * W * (N0 / A0); }
+};
+
+// The supported static geometries.  Each entry: grid (N0, N1, N2), dtype.
+#define B17_STATIC_CASES(X)                                                     \
+  X(float, 256, 256, 256, /*plane*/ 128, 16, 8, 128, 16, 8, 2, /*pencil*/ 16, 16, 8) \
+  X(float, 128, 128, 128, /*plane*/ 64, 16, 4, 128, 16, 8, 1, /*pencil*/ 16, 8, 16)  \
+  X(float, 64, 64, 64, /*plane*/ 32, 16, 2, 64, 16, 4, 1, /*pencil*/ 16, 4, 32)      \
+  X(double, 128, 128, 128, /*plane*/ 64, 16, 4, 128, 16, 8, 1, /*pencil*/ 16, 8, 8)  \
+  X(double, 64, 64, 64, /*plane*/ 32, 16, 2, 64, 16, 4, 1, /*pencil*/ 16, 4, 16)
+
+bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g) {
+#define B17_MATCH(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                         \
+  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {             \
+    if (g) {                                                                            \
+      g->p = P;                                                                         \
+      g->w = W;                                                                         \
+      g->plane_threads = PlaneS<T, M, R1, R2, Q, C1, C2, P>::threads();                 \
+      g->plane_lds_fwd = PlaneS<T, M, R1, R2, Q, C1, C2, P>::lds_fwd();                 \
+      g->plane_lds_inv = PlaneS<T, M, R1, R2, Q, C1, C2, P>::lds_inv();                 \
+      g->pencil_threads = PencilS<T, A, A0, B0, P, W>::threads();                       \
+      g->pencil_lds = PencilS<T, A, A0, B0, P, W>::lds();                               \
+    }                                                                                   \
+    return true;                                                                        \
+  }
+  B17_STATIC_CASES(B17_MATCH)
+#undef B17_MATCH
+  return false;
+}
+
+hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void* in, void* out,
+                         void* ws, const PencilParams& pc, const void* twr, const void* twc,
+                         int rs, hipStream_t st, hipEvent_t* ev) {
+  hipError_t e = hipErrorInvalidValue;
+  auto mark = [&](int i) {
+    if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
+  };
+#define B17_RUN(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                               \
+  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                   \
+    using PS = PlaneS<T, M, R1, R2, Q, C1, C2, P>;                                            \
+    using KS = PencilS<T, A, A0, B0, P, W>;                                                   \
+    auto kf = k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P>;                                      \
+    auto ki = k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P>;                                      \
+    auto kp = mode == 0 ? k_pencil_s<T, A, A0, B0, P, W, 0> : k_pencil_s<T, A, A0, B0, P, W, 1>; \
+    e = set_lds_s(kf, PS::lds_fwd());                                                         \
+    if (e == hipSuccess) e = set_lds_s(ki, PS::lds_inv());                                    \
+    if (e == hipSuccess) e = set_lds_s(kp, KS::lds());                                        \
+    mark(0);                                                                                  \
+    const int nplane = 3 * A * P;                                                             \
+    if (e == hipSuccess) {                                                                    \
+      hipLaunchKernelGGL(kf, dim3(nplane), dim3(PS::threads()), PS::lds_fwd(), st,            \
+                         (const T*)in, (cpx<T>*)ws, (const cpx<T>*)twr, (const cpx<T>*)twc, rs); \
+      e = hipGetLastError();                                                                  \
+    }                                                                                         \
+    mark(1);                                                                                  \
+    if (e == hipSuccess) {                                                                    \
+      hipLaunchKernelGGL(kp, dim3(pc.ntiles), dim3(KS::threads()), KS::lds(), st, (cpx<T>*)ws, pc); \
+      e = hipGetLastError();                                                                  \
+    }                                                                                         \
+    mark(2);                                                                                  \
+    if (e == hipSuccess) {                                                                    \
+      hipLaunchKernelGGL(ki, dim3(nplane), dim3(PS::threads()), PS::lds_inv(), st,            \
+                         (const cpx<T>*)ws, (T*)out, (const cpx<T>*)twr, (const cpx<T>*)twc, rs); \
+      e = hipGetLastError();                                                                  \
+    }                                                                                         \
+    mark(3);                                                                                  \
+    return e;                                                                                 \
+  }
+  B17_STATIC_CASES(B17_RUN)
+#undef B17_RUN
+  return e;
+}
+
+}  // namespace b17

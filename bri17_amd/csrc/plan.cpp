@@ -9,6 +9,8 @@
 // mu and nu enter the kernels as scalars.
 #include "plan.hpp"
 
+#include "kernels_static.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -133,8 +135,18 @@ static const std::vector<int>& pick(const Tuning& t, int role, const std::vector
   return t.radices[role].empty() ? dflt : t.radices[role];
 }
 
-Status build_plan(int dim, const int* shape, const double* L, int dtype, const Tuning& tun,
+Status build_plan(int dim, const int* shape, const double* L, int dtype, const Tuning& tun_in,
                   std::shared_ptr<Plan>& out) {
+  Tuning tun = tun_in;
+  StaticGeom sg;
+  const bool use_static = dim == 3 && !tun.force_generic && tun.radices[0].empty() &&
+                          tun.radices[1].empty() && tun.radices[2].empty() &&
+                          static_supported(dtype, shape[0], shape[1], shape[2], &sg);
+  if (use_static) {
+    // the generic description shares the workspace layout of the fixed-size path
+    tun.plane_split = sg.p;
+    tun.pencil_width = sg.w;
+  }
   if (dim != 2 && dim != 3) return fail(1, "dim must be 2 or 3");
   if (dtype != 0 && dtype != 1) return fail(1, "dtype must be BRI17_F32 or BRI17_F64");
   auto pl = std::make_shared<Plan>();
@@ -264,6 +276,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     pl->pencil_lds = per_w * W;
   }
   pl->ws_bytes = (size_t)dim * (nn / nl) * nlh * csz;
+  pl->use_static = use_static;
 
   Status s = dtype == 0 ? upload_tables<float>(*pl) : upload_tables<double>(*pl);
   if (!s.ok()) return s;
@@ -275,6 +288,37 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   return Status{};
 }
 
+static std::string radices_json(const FftDesc& d) {
+  std::ostringstream o;
+  o << "[";
+  for (int i = 0; i < d.npass; i++) o << (i ? "," : "") << d.pass[i].r;
+  o << "]";
+  return o.str();
+}
+
+std::string describe_plan(const Plan& pl) {
+  std::ostringstream o;
+  const PlaneParams& pp = pl.plane;
+  const PencilParams& pc = pl.pencil;
+  o << "{\"dim\":" << pl.dim << ",\"fixed_size_kernels\":" << (pl.use_static ? "true" : "false")
+    << ",\"dtype\":\"" << (pl.dtype == 0 ? "f32" : "f64") << "\""
+    << ",\"shape\":[" << pl.shape[0] << "," << pl.shape[1];
+  if (pl.dim == 3) o << "," << pl.shape[2];
+  o << "],\"plane\":{\"split\":" << pp.p << ",\"rows\":" << pp.q << ",\"blocks\":" << pl.plane_blocks
+    << ",\"threads\":" << pl.plane_threads << ",\"lds\":" << pl.plane_lds
+    << ",\"row_fft\":{\"n\":" << pp.row.n << ",\"tpt\":" << pp.row.tpt
+    << ",\"radices\":" << radices_json(pp.row) << "}";
+  if (pp.has_col)
+    o << ",\"mid_fft\":{\"n\":" << pp.col.n << ",\"tpt\":" << pp.col.tpt
+      << ",\"radices\":" << radices_json(pp.col) << "}";
+  o << "},\"pencil\":{\"width\":" << pc.w << ",\"tiles\":" << pc.ntiles
+    << ",\"threads\":" << pl.pencil_threads << ",\"lds\":" << pl.pencil_lds
+    << ",\"xcd_remap\":" << pc.xcd_remap << ",\"axis0_fft\":{\"n\":" << pc.ax0.n
+    << ",\"tpt\":" << pc.ax0.tpt << ",\"radices\":" << radices_json(pc.ax0) << "}}"
+    << ",\"workspace_bytes\":" << pl.ws_bytes << "}";
+  return o.str();
+}
+
 Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
                     void* ws, hipStream_t st, hipEvent_t* ev) {
   PencilParams pc = pl.pencil;
@@ -284,6 +328,12 @@ Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* 
   const double N = (double)pl.nnodes;
   pc.scale = mode == 0 ? pl.cell_volume / N : 1.0 / (pl.cell_volume * N);
   hipError_t e = hipSuccess;
+  if (pl.use_static) {
+    e = launch_static(pl.dtype, pl.shape[0], pl.shape[1], pl.shape[2], mode, in, out, ws, pc,
+                      pl.plane.tw_row, pl.plane.tw_col, pl.plane.rs, st, ev);
+    if (e != hipSuccess) return fail(3, std::string("kernel launch: ") + hipGetErrorString(e));
+    return Status{};
+  }
   auto mark = [&](int i) {
     if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
   };

@@ -28,6 +28,7 @@ struct Tuning {
   int pencil_width = 0;      // pencil tile width W; 0 = automatic
   size_t lds_budget = 160 * 1024;
   int xcd_remap = 1;
+  int force_generic = 0;        // 1: never use the fixed-size kernels
   std::vector<int> radices[3];  // forced radix orders per FFT length role
 };
 
@@ -56,6 +57,7 @@ struct Plan {
   int pencil_threads = 0;
   size_t pencil_lds = 0;
   size_t ws_bytes = 0;
+  bool use_static = false;  // fixed-size kernels (kernels_static.hip)
   Tuning tuning;
 
   ~Plan();
@@ -63,6 +65,9 @@ struct Plan {
 
 Status build_plan(int dim, const int* shape, const double* L, int dtype,
                   const Tuning& tun, std::shared_ptr<Plan>& out);
+
+// JSON summary of a plan's geometry (for logs and benchmarks).
+std::string describe_plan(const Plan& pl);
 
 // Launch the three-kernel pipeline.  mode 0: K u, mode 1: K^-1 f.
 // `ev`, if not null, holds 4 events recorded before the first kernel and after

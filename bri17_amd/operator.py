@@ -18,7 +18,7 @@ _DTYPES = {"float32": _lib.BRI17_F32, "float64": _lib.BRI17_F64}
 
 
 class StiffnessMatrix:
-    def __init__(self, shape, L=None, mu=1.0, nu=0.3, dtype="float32"):
+    def __init__(self, shape, L=None, mu=1.0, nu=0.3, dtype="float32", tuning=None):
         shape = tuple(int(n) for n in shape)
         if len(shape) not in (2, 3):
             raise ValueError("shape must have 2 or 3 entries")
@@ -37,8 +37,14 @@ class StiffnessMatrix:
         h = ctypes.c_void_p()
         sh = (ctypes.c_int * 3)(*(shape + (1,) * (3 - self.dim)))
         LL = (ctypes.c_double * 3)(*(L + (1.0,) * (3 - self.dim)))
-        _lib.check(_lib.lib().bri17_operator_create(self.dim, sh, LL, self._mu, self._nu,
-                                                    _DTYPES[dtype], ctypes.byref(h)))
+        if tuning is None:
+            _lib.check(_lib.lib().bri17_operator_create(self.dim, sh, LL, self._mu, self._nu,
+                                                        _DTYPES[dtype], ctypes.byref(h)))
+        else:
+            t = tuning if isinstance(tuning, _lib.Tuning) else _lib.Tuning.make(**tuning)
+            _lib.check(_lib.lib().bri17_operator_create_tuned(self.dim, sh, LL, self._mu, self._nu,
+                                                              _DTYPES[dtype], ctypes.byref(t),
+                                                              ctypes.byref(h)))
         self._h = h
         n = ctypes.c_int64()
         ws = ctypes.c_size_t()
@@ -69,6 +75,13 @@ class StiffnessMatrix:
             self.close()
         except Exception:
             pass
+
+    def describe(self):
+        """Plan geometry as a dict (kernel launch shapes, radix orders)."""
+        import json
+        buf = ctypes.create_string_buffer(4096)
+        _lib.check(_lib.lib().bri17_operator_describe(self._h, buf, len(buf)))
+        return json.loads(buf.value.decode())
 
     @property
     def handle(self):

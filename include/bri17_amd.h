@@ -88,6 +88,30 @@ int bri17_operator_create(int dim, const int* shape, const double* L, double mu,
                           double nu, int dtype, bri17_operator** out);
 int bri17_operator_destroy(bri17_operator* op);
 
+/* Launch-shape choices of a plan (no reference counterpart: FFTW chooses its
+ * own plan inside fftw_plan_dft, tests/test_bri17.cpp:119-127).  Zero fields
+ * mean "automatic". */
+typedef struct bri17_tuning {
+  int plane_split;    /* 3D: split P of the middle axis between the plane and
+                         pencil kernels (1,2,3,4,5,7,8,16) */
+  int pencil_width;   /* contiguous spectrum entries per pencil tile */
+  int lds_budget_kb;  /* LDS per workgroup, KiB (<= 160) */
+  int xcd_remap;      /* 0: off, 1: on, -1: automatic (on) */
+  int radices[3][8];  /* radix order of the row / middle-axis / axis-0
+                         transforms, 0-terminated (radices 2,3,4,5,7,8,16) */
+  int force_generic;  /* 1: use the runtime-size kernels even where a
+                         fixed-size instantiation exists */
+} bri17_tuning;
+
+/* bri17_operator_create with explicit tuning (NULL = automatic). */
+int bri17_operator_create_tuned(int dim, const int* shape, const double* L, double mu,
+                                double nu, int dtype, const bri17_tuning* tuning,
+                                bri17_operator** out);
+
+/* JSON description of the operator's plan (kernel geometry, radix orders,
+ * LDS sizes) written to buf (truncated to len-1 bytes, NUL-terminated). */
+int bri17_operator_describe(const bri17_operator* op, char* buf, size_t len);
+
 /* Change the material of an existing operator (tables are mu/nu independent). */
 int bri17_operator_set_material(bri17_operator* op, double mu, double nu);
 

@@ -65,7 +65,34 @@ CASES = [
     ((243, 12), (1.0, 1.0)),
     ((512, 96), (1.0, 0.2)),
     ((1, 6, 5), (1.0, 1.0, 1.0)),
+    ((64, 64, 64), (1.0, 1.0, 1.0)),    # fixed-size kernels (fp32 and fp64)
 ]
+
+
+@pytest.mark.parametrize("shape,dtype", [((64, 64, 64), "float32"), ((128, 128, 128), "float32"),
+                                         ((256, 256, 256), "float32"), ((64, 64, 64), "float64"),
+                                         ((128, 128, 128), "float64")])
+@pytest.mark.parametrize("mode", ["apply", "solve"])
+def test_fixed_size_kernels_match_generic(device, shape, dtype, mode):
+    """The compile-time-size kernels and the runtime-size kernels implement the
+    same pipeline on the same workspace layout: their results agree to
+    rounding, on both K u and the Green solve."""
+    import torch
+    fast = _op(shape, (1.0, 1.3, 0.7), 1.3, 0.27, dtype)
+    gen = _op(shape, (1.0, 1.3, 0.7), 1.3, 0.27, dtype)
+    gen.close()
+    from bri17_amd import StiffnessMatrix
+    gen = StiffnessMatrix(shape, (1.0, 1.3, 0.7), 1.3, 0.27, dtype, tuning=dict(force_generic=1))
+    assert fast.describe()["fixed_size_kernels"] is True
+    assert gen.describe()["fixed_size_kernels"] is False
+    g = torch.Generator(device=device).manual_seed(11)
+    x = torch.randn((3,) + shape, generator=g, device=device, dtype=getattr(torch, dtype))
+    a = fast.apply(x) if mode == "apply" else fast.solve(x)
+    b = gen.apply(x) if mode == "apply" else gen.solve(x)
+    torch.cuda.synchronize()
+    err = float((a - b).norm() / b.norm())
+    print(shape, dtype, mode, "fixed vs generic %.2e" % err)
+    assert err <= (2e-6 if dtype == "float32" else 1e-13)
 
 
 def _field(oracle, seed, shape, dist="uniform"):
