@@ -59,13 +59,15 @@ class Tuning(ctypes.Structure):
     """bri17_tuning of include/bri17_amd.h."""
     _fields_ = [("plane_split", ctypes.c_int), ("pencil_width", ctypes.c_int),
                 ("lds_budget_kb", ctypes.c_int), ("xcd_remap", ctypes.c_int),
-                ("radices", (ctypes.c_int * 8) * 3), ("force_generic", ctypes.c_int)]
+                ("radices", (ctypes.c_int * 8) * 3), ("force_generic", ctypes.c_int),
+                ("plane_prefetch", ctypes.c_int)]
 
     @classmethod
     def make(cls, plane_split=0, pencil_width=0, lds_budget_kb=0, xcd_remap=-1, radices=(None, None, None),
-             force_generic=0):
+             force_generic=0, plane_prefetch=-1):
         t = cls()
         t.force_generic = force_generic
+        t.plane_prefetch = plane_prefetch
         t.plane_split, t.pencil_width, t.lds_budget_kb, t.xcd_remap = plane_split, pencil_width, lds_budget_kb, xcd_remap
         for r, rs in enumerate(radices):
             for i, v in enumerate(rs or ()):

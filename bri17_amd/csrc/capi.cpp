@@ -342,6 +342,7 @@ int bri17_operator_create_tuned(int dim, const int* shape, const double* L, doub
     if (tuning->lds_budget_kb > 0) tun.lds_budget = (size_t)tuning->lds_budget_kb * 1024;
     if (tuning->xcd_remap >= 0) tun.xcd_remap = tuning->xcd_remap;
     tun.force_generic = tuning->force_generic;
+    if (tuning->plane_prefetch >= 0) tun.plane_prefetch = tuning->plane_prefetch;
     for (int r = 0; r < 3; r++)
       for (int i = 0; i < 8 && tuning->radices[r][i] > 0; i++)
         tun.radices[r].push_back(tuning->radices[r][i]);

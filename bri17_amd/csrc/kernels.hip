@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
   }
 
   {
-    const T mu = T(pp.mu), sc = T(pp.s), scale = T(pp.scale);
+    const T mu = T(pp.mu), sc = T(pp.s);
     const int items = N0 * PW;
     for (int e = tid; e < items; e += nthr) {
       const int k0 = (int)fdiv((uint32_t)e, pp.pwdiv);
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
       cpx<T> x[DIM];
 #pragma unroll
       for (int c = 0; c < DIM; c++) x[c] = lds[c * cst + e];
-      modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, scale, x);
+      modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, x);
 #pragma unroll
       for (int c = 0; c < DIM; c++) lds[c * cst + e] = x[c];
     }

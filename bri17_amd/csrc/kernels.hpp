@@ -62,7 +62,8 @@ struct PencilParams {
   int ntiles;
   int mode;     // 0: K u, 1: K^-1 f
   int xcd_remap;
-  double mu, s, scale;
+  double mu, s;   // material constants with the apply / solve scale folded in
+  double scale;   // |h|/|N| (apply) or 1/(|h||N|) (solve), for reference
   FastDiv wdiv, pwdiv, rsdiv;
   FftDesc ax0;
   const void* tw0;      // exp(-2 pi i j / n0)

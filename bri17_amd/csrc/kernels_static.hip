@@ -8,6 +8,11 @@
 // into VGPRs, one LDS exchange feeds the second pass, and the row -> column
 // transposition is the only other trip through LDS.  With all sizes known at
 // compile time the index arithmetic folds to shifts and constants.
+//
+// The plane kernels are persistent (one workgroup per CU slot looping over
+// planes): the global loads of the next plane are issued into the registers
+// freed by the first FFT pass, so they are in flight while the rest of the
+// current plane is computed.
 #include "fft_core.hpp"
 #include "kernels.hpp"
 #include "kernels_static.hpp"
@@ -36,6 +41,14 @@ __device__ __forceinline__ cpx<T> s_pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
   return {e.x - o.y, e.y + o.x};
 }
 
+// A copy of v the optimiser cannot see through: indices derived from it are
+// recomputed inside a persistent loop instead of being hoisted into hundreds
+// of live registers.
+__device__ __forceinline__ int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
 template <typename T>
 __device__ __forceinline__ void copy_table(cpx<T>* dst, const cpx<T>* src, int n, int tid,
                                            int nthr) {
@@ -44,15 +57,27 @@ __device__ __forceinline__ void copy_table(cpx<T>* dst, const cpx<T>* src, int n
 
 // ------------------------------------------------------------------ plane fwd
 //
-// Workgroup (c, i0, p): rows i1 = q P + p, q < Q, of M complex (= 2M reals).
+// Work item (c, i0, p): rows i1 = q P + p, q < Q, of M complex (= 2M reals).
 // Row FFT M = R1 x R2 by M/R1 threads per row; column stage Q = C1 x C2 by
 // Q/C1 threads per column on M column sequences (columns 0 and M, both real
 // after the row transform, are packed into one complex column).
 
-template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
+template <typename T, int M, int R1, int Q, int P>
+__device__ __forceinline__ void plane_load_rows(cpx<T>* v, const T* __restrict__ u, int item,
+                                                int tid) {
+  constexpr int TPR = M / R1;
+  const int plane = item / P, p = item - (item / P) * P;
+  const int q = tid / TPR, j = tid - (tid / TPR) * TPR;
+  const cpx<T>* src =
+      reinterpret_cast<const cpx<T>*>(u + ((int64_t)plane * (P * Q) + q * P + p) * (2 * M));
+#pragma unroll
+  for (int k = 0; k < R1; k++) v[k] = src[j + k * TPR];
+}
+
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
 __global__ void __launch_bounds__(Q * M / R1)
     k_plane_fwd_s(const T* __restrict__ u, cpx<T>* __restrict__ ws, const cpx<T>* __restrict__ g_twr,
-                  const cpx<T>* __restrict__ g_twc, int rs) {
+                  const cpx<T>* __restrict__ g_twc, int rs, int nitems) {
   constexpr int NT = Q * M / R1;
   constexpr int TPR = M / R1;
   constexpr int TPC = Q / C1;
@@ -66,22 +91,27 @@ __global__ void __launch_bounds__(Q * M / R1)
   cpx<T>* twc = twr + 2 * M;             // w_{N1}^j, j < N1
   cpx<T>* scr = twc + N1;                // packed-column scratch, Q entries
   const int tid = threadIdx.x;
-  const int plane = blockIdx.x / P;
-  const int p = blockIdx.x - plane * P;
 
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
 
-  // ---- rows: pass 1 straight from global memory
-  {
-    const int q = tid / TPR, j = tid - (tid / TPR) * TPR;
-    const cpx<T>* src = reinterpret_cast<const cpx<T>*>(u + ((int64_t)plane * N1 + q * P + p) * (2 * M));
-    cpx<T> v[R1];
-#pragma unroll
-    for (int k = 0; k < R1; k++) v[k] = src[j + k * TPR];
+  cpx<T> v[R1];
+  int item = blockIdx.x;
+  if (PF && item < nitems) plane_load_rows<T, M, R1, Q, P>(v, u, item, tid);
+  for (; item < nitems; item += gridDim.x) {
+    const int tl = opaque(tid);
+    if (!PF) plane_load_rows<T, M, R1, Q, P>(v, u, item, tl);
+    const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
+    const int cs = tl % M, l = tl / M;
+    const int plane = item / P;
+    const int p = item - plane * P;
+    // ---- rows: pass 1 on the loaded registers
     bfly<R1, false>(v);
 #pragma unroll
     for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
+    // prefetch the next work item into the now free registers
+    if (PF && item + (int)gridDim.x < nitems)
+      plane_load_rows<T, M, R1, Q, P>(v, u, item + gridDim.x, tl);
     __syncthreads();
     // pass 2: R1/R2 butterflies per thread, natural-order output
     cpx<T> w[R1 / R2][R2];
@@ -101,79 +131,95 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
       for (int k = 0; k < R2; k++) buf[pd(q * RS + j + b * TPR + k * R1)] = w[b][k];
     __syncthreads();
-  }
 
-  // ---- columns: pass 1 reads the split spectrum of two rows' worth of Z
-  const int cs = tid % M, l = tid / M;
-  cpx<T> x[C1];
+    // ---- columns: pass 1 reads Z of two columns and forms the spectrum
+    cpx<T> x[C1];
 #pragma unroll
-  for (int kk = 0; kk < C1; kk++) {
-    const int qq = l + kk * TPC;
-    const cpx<T> A = buf[pd(qq * RS + cs)];
-    if (cs == 0) {
-      // X[q][0] = Re Z + Im Z, X[q][M] = Re Z - Im Z, packed as X0 + i XM
-      x[kk] = {A.x + A.y, A.x - A.y};
-    } else {
-      x[kk] = s_post_r2c(A, buf[pd(qq * RS + M - cs)], twr[cs]);
-    }
-  }
-  bfly<C1, false>(x);
-  __syncthreads();
-  // column exchange layout: element idx of column cs at pd(idx * M + cs)
-#pragma unroll
-  for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
-  __syncthreads();
-  cpx<T> y[C1 / C2][C2];
-#pragma unroll
-  for (int b = 0; b < C1 / C2; b++) {
-    const int jj = l + b * TPC;
-#pragma unroll
-    for (int k = 0; k < C2; k++) {
-      y[b][k] = buf[pd((jj + k * C1) * M + cs)];
-      if (k) y[b][k] = cmul(y[b][k], twc[P * jj * k]);
-    }
-    bfly<C2, false>(y[b]);
-  }
-  // packed column: Y0[ka] = (Z[ka] + conj Z[-ka])/2, YM[ka] = -i (Z[ka] - conj Z[-ka])/2
-  if (cs == 0) {
-#pragma unroll
-    for (int b = 0; b < C1 / C2; b++)
-#pragma unroll
-      for (int k = 0; k < C2; k++) scr[l + b * TPC + k * C1] = y[b][k];
-  }
-  __syncthreads();
-  cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * rs;
-#pragma unroll
-  for (int b = 0; b < C1 / C2; b++) {
-#pragma unroll
-    for (int k = 0; k < C2; k++) {
-      const int ka = l + b * TPC + k * C1;
-      cpx<T> v = y[b][k];
+    for (int kk = 0; kk < C1; kk++) {
+      const int qq = l + kk * TPC;
+      const cpx<T> A = buf[pd(qq * RS + cs)];
       if (cs == 0) {
-        const cpx<T> zn = scr[(Q - ka) % Q];
-        cpx<T> y0 = {T(0.5) * (v.x + zn.x), T(0.5) * (v.y - zn.y)};
-        cpx<T> ym = {T(0.5) * (v.y + zn.y), T(0.5) * (zn.x - v.x)};
-        if (P > 1 && p) {
-          const cpx<T> tw = twc[p * ka];
-          y0 = cmul(y0, tw);
-          ym = cmul(ym, tw);
-        }
-        dst[(int64_t)ka * rs] = y0;
-        dst[(int64_t)ka * rs + M] = ym;
+        // X[q][0] = Re Z + Im Z, X[q][M] = Re Z - Im Z, packed as X0 + i XM
+        x[kk] = {A.x + A.y, A.x - A.y};
       } else {
-        if (P > 1 && p) v = cmul(v, twc[p * ka]);
-        dst[(int64_t)ka * rs + cs] = v;
+        x[kk] = s_post_r2c(A, buf[pd(qq * RS + M - cs)], twr[cs]);
       }
     }
+    bfly<C1, false>(x);
+    __syncthreads();
+    // column exchange layout: element idx of column cs at pd(idx * M + cs)
+#pragma unroll
+    for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
+    __syncthreads();
+    cpx<T> y[C1 / C2][C2];
+#pragma unroll
+    for (int b = 0; b < C1 / C2; b++) {
+      const int jj = l + b * TPC;
+#pragma unroll
+      for (int k = 0; k < C2; k++) {
+        y[b][k] = buf[pd((jj + k * C1) * M + cs)];
+        if (k) y[b][k] = cmul(y[b][k], twc[P * jj * k]);
+      }
+      bfly<C2, false>(y[b]);
+    }
+    // packed column: Y0 = (Z[ka] + conj Z[-ka])/2, YM = -i (Z[ka] - conj Z[-ka])/2
+    if (cs == 0) {
+#pragma unroll
+      for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+        for (int k = 0; k < C2; k++) scr[l + b * TPC + k * C1] = y[b][k];
+    }
+    __syncthreads();
+    cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * rs;
+#pragma unroll
+    for (int b = 0; b < C1 / C2; b++) {
+#pragma unroll
+      for (int k = 0; k < C2; k++) {
+        const int ka = l + b * TPC + k * C1;
+        cpx<T> val = y[b][k];
+        if (cs == 0) {
+          const cpx<T> zn = scr[(Q - ka) % Q];
+          cpx<T> y0 = {T(0.5) * (val.x + zn.x), T(0.5) * (val.y - zn.y)};
+          cpx<T> ym = {T(0.5) * (val.y + zn.y), T(0.5) * (zn.x - val.x)};
+          if (P > 1 && p) {
+            const cpx<T> tw = twc[p * ka];
+            y0 = cmul(y0, tw);
+            ym = cmul(ym, tw);
+          }
+          dst[(int64_t)ka * rs] = y0;
+          dst[(int64_t)ka * rs + M] = ym;
+        } else {
+          if (P > 1 && p) val = cmul(val, twc[p * ka]);
+          dst[(int64_t)ka * rs + cs] = val;
+        }
+      }
+    }
+    __syncthreads();  // LDS is rewritten by the next work item
   }
 }
 
 // ------------------------------------------------------------------ plane inv
 
-template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
+template <typename T, int M, int C1, int Q, int P>
+__device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>* xm, const cpx<T>* __restrict__ ws,
+                                                int rs, int item, int tid) {
+  constexpr int TPC = Q / C1;
+  const int plane = item / P, p = item - (item / P) * P;
+  const int cs = tid % M, l = tid / M;
+  const cpx<T>* src = ws + ((int64_t)plane * (P * Q) + p * Q) * rs;
+#pragma unroll
+  for (int kk = 0; kk < C1; kk++) {
+    const int ka = l + kk * TPC;
+    x[kk] = src[(int64_t)ka * rs + cs];
+    if (cs == 0) xm[kk] = src[(int64_t)ka * rs + M];
+  }
+}
+
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
 __global__ void __launch_bounds__(Q * M / R1)
     k_plane_inv_s(const cpx<T>* __restrict__ ws, T* __restrict__ out,
-                  const cpx<T>* __restrict__ g_twr, const cpx<T>* __restrict__ g_twc, int rs) {
+                  const cpx<T>* __restrict__ g_twr, const cpx<T>* __restrict__ g_twc, int rs,
+                  int nitems) {
   constexpr int NT = Q * M / R1;
   constexpr int TPR = M / R1;
   constexpr int TPC = Q / C1;
@@ -183,103 +229,100 @@ __global__ void __launch_bounds__(Q * M / R1)
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
-  cpx<T>* twr = buf + pd_size(Q * RS > Q * M ? Q * RS : Q * M);
+  cpx<T>* twr = buf + pd_size(Q * RS);
   cpx<T>* twc = twr + 2 * M;
   const int tid = threadIdx.x;
-  const int plane = blockIdx.x / P;
-  const int p = blockIdx.x - plane * P;
 
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
+  __syncthreads();
 
-  // ---- columns: inverse Q-point stage, pass 1 straight from global memory
-  const int cs = tid % M, l = tid / M;
-  const cpx<T>* src = ws + ((int64_t)plane * N1 + p * Q) * rs;
-  cpx<T> x[C1];
+  cpx<T> x[C1], xm[C1];
+  int item = blockIdx.x;
+  if (PF && item < nitems) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item, tid);
+  for (; item < nitems; item += gridDim.x) {
+    const int tl = opaque(tid);
+    if (!PF) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item, tl);
+    const int cs = tl % M, l = tl / M;
+    const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
+    const int plane = item / P;
+    const int p = item - plane * P;
+    // ---- columns: inverse Q-point stage
 #pragma unroll
-  for (int kk = 0; kk < C1; kk++) {
-    const int ka = l + kk * TPC;
-    cpx<T> v = src[(int64_t)ka * rs + cs];
-    if (cs == 0) {
-      // columns 0 and M are both Hermitian sequences: pack as X0 + i XM
-      const cpx<T> vm = src[(int64_t)ka * rs + M];
-      v = {v.x - vm.y, v.y + vm.x};
+    for (int kk = 0; kk < C1; kk++) {
+      cpx<T> val = x[kk];
+      if (cs == 0) val = {val.x - xm[kk].y, val.y + xm[kk].x};  // X0 + i XM (both Hermitian)
+      if (P > 1 && p) val = cmulc(val, twc[p * (l + kk * TPC)]);
+      x[kk] = val;
     }
-    x[kk] = v;
-  }
-  __syncthreads();  // twiddle tables ready
-  if (P > 1 && p) {
+    bfly<C1, true>(x);
 #pragma unroll
-    for (int kk = 0; kk < C1; kk++) x[kk] = cmulc(x[kk], twc[p * (l + kk * TPC)]);
-  }
-  bfly<C1, true>(x);
+    for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
+    // prefetch the next work item
+    if (PF && item + (int)gridDim.x < nitems)
+      plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item + gridDim.x, tl);
+    __syncthreads();
+    cpx<T> y[C1 / C2][C2];
 #pragma unroll
-  for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
-  __syncthreads();
-  cpx<T> y[C1 / C2][C2];
+    for (int b = 0; b < C1 / C2; b++) {
+      const int jj = l + b * TPC;
 #pragma unroll
-  for (int b = 0; b < C1 / C2; b++) {
-    const int jj = l + b * TPC;
-#pragma unroll
-    for (int k = 0; k < C2; k++) {
-      y[b][k] = buf[pd((jj + k * C1) * M + cs)];
-      if (k) y[b][k] = cmulc(y[b][k], twc[P * jj * k]);
-    }
-    bfly<C2, true>(y[b]);
-  }
-  __syncthreads();
-  // rows q = jj + C1 k, natural order; unpack columns 0 and M
-#pragma unroll
-  for (int b = 0; b < C1 / C2; b++)
-#pragma unroll
-    for (int k = 0; k < C2; k++) {
-      const int qq = l + b * TPC + k * C1;
-      const cpx<T> v = y[b][k];
-      if (cs == 0) {
-        buf[pd(qq * RS)] = {v.x, T(0)};
-        buf[pd(qq * RS + M)] = {v.y, T(0)};
-      } else {
-        buf[pd(qq * RS + cs)] = v;
+      for (int k = 0; k < C2; k++) {
+        y[b][k] = buf[pd((jj + k * C1) * M + cs)];
+        if (k) y[b][k] = cmulc(y[b][k], twc[P * jj * k]);
       }
+      bfly<C2, true>(y[b]);
     }
-  __syncthreads();
+    __syncthreads();
+    // rows q = jj + C1 k, natural order; unpack columns 0 and M
+#pragma unroll
+    for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+      for (int k = 0; k < C2; k++) {
+        const int qq = l + b * TPC + k * C1;
+        const cpx<T> val = y[b][k];
+        if (cs == 0) {
+          buf[pd(qq * RS)] = {val.x, T(0)};
+          buf[pd(qq * RS + M)] = {val.y, T(0)};
+        } else {
+          buf[pd(qq * RS + cs)] = val;
+        }
+      }
+    __syncthreads();
 
-  // ---- rows: C2R split, then the inverse M-point transform
-  const int q = tid / TPR, j = tid - (tid / TPR) * TPR;
-  cpx<T> v[R1];
+    // ---- rows: C2R split, then the inverse M-point transform
+    cpx<T> v[R1];
 #pragma unroll
-  for (int k = 0; k < R1; k++) {
-    const int mm = j + k * TPR;
-    const cpx<T> A = buf[pd(q * RS + mm)];
-    const cpx<T> B = buf[pd(q * RS + M - mm)];
-    if (mm == 0) {
+    for (int k = 0; k < R1; k++) {
+      const int mm = j + k * TPR;
+      const cpx<T> A = buf[pd(q * RS + mm)];
+      const cpx<T> B = buf[pd(q * RS + M - mm)];
       // Z[0] = (X0 + conj XM) + i (X0 - conj XM) with X0, XM real
-      v[k] = {A.x + B.x, A.x - B.x};
-    } else {
-      v[k] = s_pre_c2r(A, B, twr[mm]);
+      v[k] = mm == 0 ? cpx<T>{A.x + B.x, A.x - B.x} : s_pre_c2r(A, B, twr[mm]);
     }
-  }
-  bfly<R1, true>(v);
-  __syncthreads();
+    bfly<R1, true>(v);
+    __syncthreads();
 #pragma unroll
-  for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
-  __syncthreads();
-  cpx<T> w[R1 / R2][R2];
+    for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
+    __syncthreads();
+    cpx<T> w[R1 / R2][R2];
 #pragma unroll
-  for (int b = 0; b < R1 / R2; b++) {
-    const int jj = j + b * TPR;
+    for (int b = 0; b < R1 / R2; b++) {
+      const int jj = j + b * TPR;
 #pragma unroll
-    for (int k = 0; k < R2; k++) {
-      w[b][k] = buf[pd(q * RS + jj + k * R1)];
-      if (k) w[b][k] = cmulc(w[b][k], twr[2 * jj * k]);
+      for (int k = 0; k < R2; k++) {
+        w[b][k] = buf[pd(q * RS + jj + k * R1)];
+        if (k) w[b][k] = cmulc(w[b][k], twr[2 * jj * k]);
+      }
+      bfly<R2, true>(w[b]);
     }
-    bfly<R2, true>(w[b]);
+    cpx<T>* dst = reinterpret_cast<cpx<T>*>(out + ((int64_t)plane * N1 + q * P + p) * (2 * M));
+#pragma unroll
+    for (int b = 0; b < R1 / R2; b++)
+#pragma unroll
+      for (int k = 0; k < R2; k++) dst[j + b * TPR + k * R1] = w[b][k];
+    __syncthreads();  // LDS is rewritten by the next work item
   }
-  cpx<T>* dst = reinterpret_cast<cpx<T>*>(out + ((int64_t)plane * N1 + q * P + p) * (2 * M));
-#pragma unroll
-  for (int b = 0; b < R1 / R2; b++)
-#pragma unroll
-    for (int k = 0; k < R2; k++) dst[j + b * TPR + k * R1] = w[b][k];
 }
 
 // ---------------------------------------------------------------------- pencil
@@ -295,6 +338,7 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
   constexpr int NT = 3 * W * TPA;
   constexpr int ROWS = 3 * P * N0;  // LDS rows of W entries: ((c P + kb) N0 + n0)
   static_assert(A0 * B0 == N0 && A0 % B0 == 0, "pencil geometry");
+  static_assert(NT % (P * W) == 0, "block mapping");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* tw = buf + pd_size(ROWS * W);
@@ -315,20 +359,19 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
   const int l = (tid / W) % TPA;
   const int c = tid / (W * TPA);
   const bool wok = w < wv;
+  // out-of-range columns of the last tile read a valid address and are never stored
+  const int fw = wok ? f0 + w : F - 1;
 
   copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, NT);
 
   // load + split DFT over p + pass 1
   cpx<T> v[P][A0];
   {
-    const cpx<T>* src = ws + (int64_t)c * N0 * P * F + f0 + w;
+    const cpx<T>* src = ws + (int64_t)c * N0 * P * F + fw;
 #pragma unroll
     for (int k = 0; k < A0; k++)
 #pragma unroll
-      for (int pp_ = 0; pp_ < P; pp_++) {
-        const int i0 = l + k * TPA;
-        v[pp_][k] = wok ? src[((int64_t)i0 * P + pp_) * F] : cpx<T>{T(0), T(0)};
-      }
+      for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = src[((int64_t)(l + k * TPA) * P + pp_) * F];
   }
   if constexpr (P > 1) {
 #pragma unroll
@@ -374,27 +417,36 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
   }
   __syncthreads();
 
-  // modal block on every point (k0, kb, w) of the tile
+  // modal block: a thread keeps (kb, w) fixed and walks k0, so the factors of
+  // axes 1 and 2 are formed once
   {
-    const T mu = T(pp.mu), sc = T(pp.s), scale = T(pp.scale);
-    constexpr int NPTS = N0 * P * W;
-    for (int e = tid; e < NPTS; e += NT) {
-      const int ww = e % W;
-      const int kb = (e / W) % P;
-      const int k0 = e / (W * P);
-      if (ww >= wv) continue;
-      const int f = f0 + ww;
-      const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
-      const int kl = f - ka * pp.rs;
-      if (kl >= pp.nlh) continue;
-      const int k1 = ka + pp.q * kb;
-      const ModalCoef<T>* cf[3] = {coef0, coef1, coef2};
-      cpx<T> xx[3];
+    constexpr int KT = NT / (P * W);  // threads per (kb, w)
+    const int bw = tid % W;
+    const int bk = (tid / W) % KT;
+    const int bkb = tid / (W * KT);
+    const int f = f0 + bw;
+    const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
+    const int kl = f - ka * pp.rs;
+    if (bw < wv && kl < pp.nlh) {
+      const int k1 = ka + pp.q * bkb;
+      const T mu = T(pp.mu), s = T(pp.s);
+      const ModalCoef<T> b = coef1[k1], cc = coef2[kl];
+      const T chichi = b.chi * cc.chi, phichi = b.phi * cc.chi, chiphi = b.chi * cc.phi;
+      const T s_psichi = s * b.psi * cc.chi, s_chipsi = s * b.chi * cc.psi;
+      const T s_psipsi = s * b.psi * cc.psi;
+      const bool kz = k1 == 0 && kl == 0;
+      for (int k0 = bk; k0 < N0; k0 += KT) {
+        const ModalCoef<T> a = coef0[k0];
+        const T H0 = a.phi * chichi, H1 = a.chi * phichi, H2 = a.chi * chiphi;
+        const T dg = mu * (H0 + H1 + H2);
+        cpx<T> xx[3];
 #pragma unroll
-      for (int cc = 0; cc < 3; cc++) xx[cc] = buf[pd(((cc * P + kb) * N0 + k0) * W + ww)];
-      modal_block<T, 3, MODE>(cf, k0, k1, kl, mu, sc, scale, xx);
+        for (int q3 = 0; q3 < 3; q3++) xx[q3] = buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)];
+        sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
+                            a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
 #pragma unroll
-      for (int cc = 0; cc < 3; cc++) buf[pd(((cc * P + kb) * N0 + k0) * W + ww)] = xx[cc];
+        for (int q3 = 0; q3 < 3; q3++) buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)] = xx[q3];
+      }
     }
   }
   __syncthreads();
@@ -477,27 +529,48 @@ struct PencilS {
   static constexpr int threads() { return 3 * W * (N0 / A0); }
 };
 
-// The supported static geometries.  Each entry: grid (N0, N1, N2), dtype.
-#define B17_STATIC_CASES(X)                                                     \
-  X(float, 256, 256, 256, /*plane*/ 128, 16, 8, 128, 16, 8, 2, /*pencil*/ 16, 16, 8) \
-  X(float, 128, 128, 128, /*plane*/ 64, 16, 4, 128, 16, 8, 1, /*pencil*/ 16, 8, 16)  \
-  X(float, 64, 64, 64, /*plane*/ 32, 16, 2, 64, 16, 4, 1, /*pencil*/ 16, 4, 32)      \
-  X(double, 128, 128, 128, /*plane*/ 64, 16, 4, 128, 16, 8, 1, /*pencil*/ 16, 8, 8)  \
+// Supported fixed-size geometries:
+//   dtype, N0, N1, N2,
+//   plane: M = N2/2 = R1 R2, Q = N1/P = C1 C2, split P,
+//   pencil: N0 = A0 B0, tile width W.
+#define B17_STATIC_CASES(X)                                                           \
+  X(float, 256, 256, 256, /*plane*/ 128, 16, 8, 128, 16, 8, 2, /*pencil*/ 16, 16, 8)  \
+  X(float, 128, 128, 128, /*plane*/ 64, 16, 4, 128, 16, 8, 1, /*pencil*/ 16, 8, 16)   \
+  X(float, 64, 64, 64, /*plane*/ 32, 16, 2, 64, 16, 4, 1, /*pencil*/ 16, 4, 32)       \
+  X(double, 128, 128, 128, /*plane*/ 64, 16, 4, 128, 16, 8, 1, /*pencil*/ 16, 8, 8)   \
   X(double, 64, 64, 64, /*plane*/ 32, 16, 2, 64, 16, 4, 1, /*pencil*/ 16, 4, 16)
 
+// persistent plane grid: workgroups resident at once on the device
+template <typename K>
+static int resident_blocks(K kernel, int threads, size_t lds, int nitems) {
+  static int ncu = 0;
+  if (!ncu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      ncu = 256;
+  }
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(kernel),
+                                                   threads, lds) != hipSuccess || per < 1)
+    per = 1;
+  const int n = ncu * per;
+  return n < nitems ? n : nitems;
+}
+
 bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g) {
-#define B17_MATCH(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                         \
-  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {             \
-    if (g) {                                                                            \
-      g->p = P;                                                                         \
-      g->w = W;                                                                         \
-      g->plane_threads = PlaneS<T, M, R1, R2, Q, C1, C2, P>::threads();                 \
-      g->plane_lds_fwd = PlaneS<T, M, R1, R2, Q, C1, C2, P>::lds_fwd();                 \
-      g->plane_lds_inv = PlaneS<T, M, R1, R2, Q, C1, C2, P>::lds_inv();                 \
-      g->pencil_threads = PencilS<T, A, A0, B0, P, W>::threads();                       \
-      g->pencil_lds = PencilS<T, A, A0, B0, P, W>::lds();                               \
-    }                                                                                   \
-    return true;                                                                        \
+#define B17_MATCH(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)             \
+  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {  \
+    if (g) {                                                                 \
+      g->p = P;                                                              \
+      g->w = W;                                                              \
+      g->plane_threads = PlaneS<T, M, R1, R2, Q, C1, C2, P>::threads();      \
+      g->plane_lds_fwd = PlaneS<T, M, R1, R2, Q, C1, C2, P>::lds_fwd();      \
+      g->plane_lds_inv = PlaneS<T, M, R1, R2, Q, C1, C2, P>::lds_inv();      \
+      g->pencil_threads = PencilS<T, A, A0, B0, P, W>::threads();            \
+      g->pencil_lds = PencilS<T, A, A0, B0, P, W>::lds();                    \
+    }                                                                        \
+    return true;                                                             \
   }
   B17_STATIC_CASES(B17_MATCH)
 #undef B17_MATCH
@@ -506,41 +579,46 @@ bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g) {
 
 hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void* in, void* out,
                          void* ws, const PencilParams& pc, const void* twr, const void* twc,
-                         int rs, hipStream_t st, hipEvent_t* ev) {
+                         int rs, int pf, hipStream_t st, hipEvent_t* ev) {
   hipError_t e = hipErrorInvalidValue;
   auto mark = [&](int i) {
     if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
   };
-#define B17_RUN(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                               \
-  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                   \
-    using PS = PlaneS<T, M, R1, R2, Q, C1, C2, P>;                                            \
-    using KS = PencilS<T, A, A0, B0, P, W>;                                                   \
-    auto kf = k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P>;                                      \
-    auto ki = k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P>;                                      \
-    auto kp = mode == 0 ? k_pencil_s<T, A, A0, B0, P, W, 0> : k_pencil_s<T, A, A0, B0, P, W, 1>; \
-    e = set_lds_s(kf, PS::lds_fwd());                                                         \
-    if (e == hipSuccess) e = set_lds_s(ki, PS::lds_inv());                                    \
-    if (e == hipSuccess) e = set_lds_s(kp, KS::lds());                                        \
-    mark(0);                                                                                  \
-    const int nplane = 3 * A * P;                                                             \
-    if (e == hipSuccess) {                                                                    \
-      hipLaunchKernelGGL(kf, dim3(nplane), dim3(PS::threads()), PS::lds_fwd(), st,            \
-                         (const T*)in, (cpx<T>*)ws, (const cpx<T>*)twr, (const cpx<T>*)twc, rs); \
-      e = hipGetLastError();                                                                  \
-    }                                                                                         \
-    mark(1);                                                                                  \
-    if (e == hipSuccess) {                                                                    \
+#define B17_RUN(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                                   \
+  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                       \
+    using PS = PlaneS<T, M, R1, R2, Q, C1, C2, P>;                                                \
+    using KS = PencilS<T, A, A0, B0, P, W>;                                                       \
+    auto kf = (pf & 1) ? k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, true>                         \
+                       : k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, false>;                        \
+    auto ki = (pf & 2) ? k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, true>                         \
+                       : k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, false>;                        \
+    auto kp = mode == 0 ? k_pencil_s<T, A, A0, B0, P, W, 0> : k_pencil_s<T, A, A0, B0, P, W, 1>;  \
+    e = set_lds_s(kf, PS::lds_fwd());                                                             \
+    if (e == hipSuccess) e = set_lds_s(ki, PS::lds_inv());                                        \
+    if (e == hipSuccess) e = set_lds_s(kp, KS::lds());                                            \
+    const int nitems = 3 * A * P;                                                                 \
+    const int gf = (pf & 1) ? resident_blocks(kf, PS::threads(), PS::lds_fwd(), nitems) : nitems; \
+    const int gi = (pf & 2) ? resident_blocks(ki, PS::threads(), PS::lds_inv(), nitems) : nitems; \
+    mark(0);                                                                                      \
+    if (e == hipSuccess) {                                                                        \
+      hipLaunchKernelGGL(kf, dim3(gf), dim3(PS::threads()), PS::lds_fwd(), st, (const T*)in,      \
+                         (cpx<T>*)ws, (const cpx<T>*)twr, (const cpx<T>*)twc, rs, nitems);        \
+      e = hipGetLastError();                                                                      \
+    }                                                                                             \
+    mark(1);                                                                                      \
+    if (e == hipSuccess) {                                                                        \
       hipLaunchKernelGGL(kp, dim3(pc.ntiles), dim3(KS::threads()), KS::lds(), st, (cpx<T>*)ws, pc); \
-      e = hipGetLastError();                                                                  \
-    }                                                                                         \
-    mark(2);                                                                                  \
-    if (e == hipSuccess) {                                                                    \
-      hipLaunchKernelGGL(ki, dim3(nplane), dim3(PS::threads()), PS::lds_inv(), st,            \
-                         (const cpx<T>*)ws, (T*)out, (const cpx<T>*)twr, (const cpx<T>*)twc, rs); \
-      e = hipGetLastError();                                                                  \
-    }                                                                                         \
-    mark(3);                                                                                  \
-    return e;                                                                                 \
+      e = hipGetLastError();                                                                      \
+    }                                                                                             \
+    mark(2);                                                                                      \
+    if (e == hipSuccess) {                                                                        \
+      hipLaunchKernelGGL(ki, dim3(gi), dim3(PS::threads()), PS::lds_inv(), st,                    \
+                         (const cpx<T>*)ws, (T*)out, (const cpx<T>*)twr, (const cpx<T>*)twc, rs,  \
+                         nitems);                                                                 \
+      e = hipGetLastError();                                                                      \
+    }                                                                                             \
+    mark(3);                                                                                      \
+    return e;                                                                                     \
   }
   B17_STATIC_CASES(B17_RUN)
 #undef B17_RUN

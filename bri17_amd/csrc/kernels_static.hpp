@@ -19,8 +19,9 @@ struct StaticGeom {
 bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g);
 
 // Launch the three fixed-size kernels (ev: optional 4 timing events).
+// pf bit 0 / bit 1: persistent prefetching forward / inverse plane kernel.
 hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void* in, void* out,
                          void* ws, const PencilParams& pc, const void* twr, const void* twc,
-                         int rs, hipStream_t st, hipEvent_t* ev);
+                         int rs, int pf, hipStream_t st, hipEvent_t* ev);
 
 }  // namespace b17

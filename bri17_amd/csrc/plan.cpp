@@ -322,15 +322,20 @@ std::string describe_plan(const Plan& pl) {
 Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
                     void* ws, hipStream_t st, hipEvent_t* ev) {
   PencilParams pc = pl.pencil;
-  pc.mu = mu;
-  pc.s = mu / (1.0 - 2.0 * nu);
   pc.mode = mode;
   const double N = (double)pl.nnodes;
+  // apply: K u scaled by |h|/|N| (tests/test_bri17.cpp:98-106); solve: the
+  // inverse scaled by 1/(|h||N|).  The factor is folded into mu and
+  // s = mu/(1-2nu) (bri17.hpp:266): c K(mu, s) = K(c mu, c s).
   pc.scale = mode == 0 ? pl.cell_volume / N : 1.0 / (pl.cell_volume * N);
+  const double s = mu / (1.0 - 2.0 * nu);
+  pc.mu = mode == 0 ? mu * pc.scale : mu / pc.scale;
+  pc.s = mode == 0 ? s * pc.scale : s / pc.scale;
   hipError_t e = hipSuccess;
   if (pl.use_static) {
     e = launch_static(pl.dtype, pl.shape[0], pl.shape[1], pl.shape[2], mode, in, out, ws, pc,
-                      pl.plane.tw_row, pl.plane.tw_col, pl.plane.rs, st, ev);
+                      pl.plane.tw_row, pl.plane.tw_col, pl.plane.rs, pl.tuning.plane_prefetch, st,
+                      ev);
     if (e != hipSuccess) return fail(3, std::string("kernel launch: ") + hipGetErrorString(e));
     return Status{};
   }

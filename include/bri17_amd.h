@@ -101,6 +101,9 @@ typedef struct bri17_tuning {
                          transforms, 0-terminated (radices 2,3,4,5,7,8,16) */
   int force_generic;  /* 1: use the runtime-size kernels even where a
                          fixed-size instantiation exists */
+  int plane_prefetch; /* fixed-size plane kernels run persistently with the
+                         next plane's loads in flight: bit 0 forward, bit 1
+                         inverse; -1 = automatic */
 } bri17_tuning;
 
 /* bri17_operator_create with explicit tuning (NULL = automatic). */
