@@ -59,6 +59,11 @@ def main(ref_root="/root/reference"):
     with open(os.path.join(HERE, "reference_fixtures.json"), "w") as f:
         json.dump(data, f, indent=0)
         f.write("\n")
+    # the same matrices as plain text (row-major, one value per line) for the
+    # C++ drop-in test, which has no JSON parser
+    for name in ("Ke2", "Ke3", "Be2", "Be3"):
+        with open(os.path.join(HERE, name + ".txt"), "w") as f:
+            f.write("\n".join(repr(v) for v in data[name]) + "\n")
 
 
 if __name__ == "__main__":
