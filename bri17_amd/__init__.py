@@ -5,7 +5,8 @@ ABI of include/bri17_amd.h).  Importing the package does not touch the GPU.
 """
 from . import _lib
 from .operator import StiffnessMatrix
+from .batch import Batch
 from . import pybri17
 
-__all__ = ["StiffnessMatrix", "pybri17"]
+__all__ = ["StiffnessMatrix", "Batch", "pybri17"]
 __version__ = "0.1.0"

@@ -49,10 +49,22 @@ SIGNATURES = {
     "bri17_operator_create_tuned": (
         _c.c_int, [_c.c_int, _ip, _dp, _c.c_double, _c.c_double, _c.c_int, _vp, _c.POINTER(_vp)]),
     "bri17_operator_describe": (_c.c_int, [_vp, _c.c_char_p, _c.c_size_t]),
+    "bri17_batch_create": (_c.c_int, [_c.POINTER(_vp)]),
+    "bri17_batch_destroy": (_c.c_int, [_vp]),
+    "bri17_batch_run": (_c.c_int, [_vp, _vp, _c.c_int, _vp]),
+    "bri17_batch_wait": (_c.c_int, [_vp, _ip]),
+    "bri17_batch_set_spin_limit": (_c.c_int, [_vp, _c.c_int]),
+    "bri17_batch_info": (_c.c_int, [_vp, _ip, _ip, _ip, _c.POINTER(_c.c_size_t)]),
     "bri17_operator_profile": (_c.c_int, [_vp, _c.c_int]),
     "bri17_operator_profile_read": (_c.c_int, [_vp, _ip, _dp]),
 }
 
+
+
+class BatchItem(ctypes.Structure):
+    """bri17_batch_item of include/bri17_amd.h."""
+    _fields_ = [("op", ctypes.c_void_p), ("mode", ctypes.c_int), ("inp", ctypes.c_void_p),
+                ("out", ctypes.c_void_p)]
 
 
 class Tuning(ctypes.Structure):

@@ -1,0 +1,28 @@
+// Persistent batched launch over many operators (see batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace b17 {
+
+// Device-side descriptor of one batch instance.
+struct BatchInst {
+  PlaneParams pp;
+  PencilParams pc;  // mu and s already folded with the mode's scale
+  const void* in;
+  void* out;
+  void* ws;
+  int na;  // plane blocks (forward and inverse each)
+  int nb;  // pencil tiles
+};
+
+// Work item encoding: x = inst | type << 16 (0 plane fwd, 1 pencil,
+// 2 plane inv), y = block / tile index within the instance.
+// ctl: [head, error, done(2 per instance)], nctl = 2 + 2 * ninst.
+hipError_t launch_batch(int dtype, const BatchInst* insts, const int2* items, int nitems, int* ctl,
+                        int nctl, int threads, size_t lds, int spin_limit, hipStream_t st,
+                        int* grid_out);
+
+}  // namespace b17

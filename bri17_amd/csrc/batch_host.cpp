@@ -1,0 +1,190 @@
+// Host side of the persistent batched launch (bri17_batch_* of
+// include/bri17_amd.h): builds the instance table and the work queue of a
+// batch of operators, uploads them with the launch and tracks completion.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <set>
+#include <vector>
+
+#include "batch.hpp"
+#include "bri17_amd.h"
+#include "capi_internal.hpp"
+
+struct bri17_batch {
+  void* d_buf = nullptr;  // device: instances | items | control words
+  size_t d_cap = 0;
+  void* h_buf = nullptr;  // pinned host staging of the same layout
+  size_t h_cap = 0;
+  hipEvent_t done = nullptr;  // recorded after the last launch
+  bool pending = false;
+  int* d_ctl = nullptr;
+  int nctl = 0;
+  int spin_limit = 1 << 24;
+  int last_nitems = 0, last_grid = 0, last_threads = 0;
+  size_t last_lds = 0;
+  std::mutex mtx;
+};
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int wait_pending(bri17_batch* b) {
+  if (b->pending) {
+    if (hipEventSynchronize(b->done) != hipSuccess)
+      return b17_err(BRI17_ERR_HIP, "hipEventSynchronize(batch) failed");
+    b->pending = false;
+  }
+  return BRI17_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bri17_batch_create(bri17_batch** out) {
+  if (!out) return b17_err(BRI17_ERR_INVALID, "null argument");
+  auto* b = new (std::nothrow) bri17_batch;
+  if (!b) return b17_err(BRI17_ERR_NOMEM, "out of host memory");
+  if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+    delete b;
+    return b17_err(BRI17_ERR_HIP, "hipEventCreate failed");
+  }
+  *out = b;
+  return BRI17_OK;
+}
+
+int bri17_batch_destroy(bri17_batch* b) {
+  if (!b) return BRI17_OK;
+  wait_pending(b);
+  if (b->d_buf) (void)hipFree(b->d_buf);
+  if (b->h_buf) (void)hipHostFree(b->h_buf);
+  if (b->done) (void)hipEventDestroy(b->done);
+  delete b;
+  return BRI17_OK;
+}
+
+int bri17_batch_set_spin_limit(bri17_batch* b, int limit) {
+  if (!b || limit < 0) return b17_err(BRI17_ERR_INVALID, "invalid argument");
+  b->spin_limit = limit;
+  return BRI17_OK;
+}
+
+int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* stream) {
+  if (!b || !items || n < 1) return b17_err(BRI17_ERR_INVALID, "invalid argument");
+  if (n > 65535) return b17_err(BRI17_ERR_INVALID, "at most 65535 instances per batch");
+  std::lock_guard<std::mutex> g(b->mtx);
+  const int dtype = items[0].op ? items[0].op->plan->dtype : -1;
+  std::set<const bri17_operator*> seen;
+  for (int i = 0; i < n; i++) {
+    const bri17_batch_item& it = items[i];
+    if (!it.op || !it.in || !it.out || it.in == it.out)
+      return b17_err(BRI17_ERR_INVALID, "batch item with a null or aliased buffer");
+    if (it.mode != 0 && it.mode != 1) return b17_err(BRI17_ERR_INVALID, "mode must be 0 or 1");
+    if (it.op->plan->dtype != dtype)
+      return b17_err(BRI17_ERR_INVALID, "all operators of a batch must share the dtype");
+    if (!seen.insert(it.op).second)
+      return b17_err(BRI17_ERR_INVALID,
+                     "an operator appears twice in one batch (its workspace would be shared)");
+  }
+  int rc = wait_pending(b);
+  if (rc) return rc;
+
+  std::vector<b17::BatchInst> insts((size_t)n);
+  std::vector<int2> work;
+  int threads = 64;
+  size_t lds = 0;
+  for (int i = 0; i < n; i++) {
+    bri17_operator* op = items[i].op;
+    std::lock_guard<std::mutex> og(op->mtx);
+    rc = b17_ensure_ws(op);
+    if (rc) return rc;
+    const b17::Plan& pl = *op->plan;
+    b17::BatchInst& I = insts[i];
+    std::memset(&I, 0, sizeof(I));
+    I.pp = pl.plane;
+    I.pc = b17::pencil_for(pl, op->mu, op->nu, items[i].mode);
+    I.pc.xcd_remap = 0;
+    I.in = items[i].in;
+    I.out = items[i].out;
+    I.ws = op->ws;
+    I.na = pl.plane_blocks;
+    I.nb = pl.pencil.ntiles;
+    threads = std::max({threads, pl.plane_threads, pl.pencil_threads});
+    lds = std::max({lds, pl.plane_lds, pl.pencil_lds});
+  }
+  // queue: every forward plane block, then every pencil tile, then every
+  // inverse plane block -- an item depends only on items before it
+  for (int type = 0; type < 3; type++)
+    for (int i = 0; i < n; i++) {
+      const int cnt = type == 1 ? insts[i].nb : insts[i].na;
+      for (int k = 0; k < cnt; k++) work.push_back(make_int2(i | (type << 16), k));
+    }
+  const int nitems = (int)work.size();
+  const int nctl = 2 + 2 * n;
+  const size_t off_items = align_up(sizeof(b17::BatchInst) * n, 256);
+  const size_t off_ctl = align_up(off_items + sizeof(int2) * work.size(), 256);
+  const size_t bytes = off_ctl + sizeof(int) * nctl;
+  if (bytes > b->d_cap) {
+    if (b->d_buf) (void)hipFree(b->d_buf);
+    if (b->h_buf) (void)hipHostFree(b->h_buf);
+    b->d_buf = b->h_buf = nullptr;
+    b->d_cap = b->h_cap = 0;
+    if (hipMalloc(&b->d_buf, bytes) != hipSuccess)
+      return b17_err(BRI17_ERR_NOMEM, "hipMalloc(batch tables) failed");
+    if (hipHostMalloc(&b->h_buf, bytes, hipHostMallocDefault) != hipSuccess)
+      return b17_err(BRI17_ERR_NOMEM, "hipHostMalloc(batch staging) failed");
+    b->d_cap = b->h_cap = bytes;
+  }
+  char* h = static_cast<char*>(b->h_buf);
+  std::memcpy(h, insts.data(), sizeof(b17::BatchInst) * n);
+  std::memcpy(h + off_items, work.data(), sizeof(int2) * work.size());
+  hipStream_t st = (hipStream_t)stream;
+  if (hipMemcpyAsync(b->d_buf, b->h_buf, off_ctl, hipMemcpyHostToDevice, st) != hipSuccess)
+    return b17_err(BRI17_ERR_HIP, "hipMemcpyAsync(batch tables) failed");
+  char* d = static_cast<char*>(b->d_buf);
+  b->d_ctl = reinterpret_cast<int*>(d + off_ctl);
+  b->nctl = nctl;
+  int grid = 0;
+  hipError_t e = b17::launch_batch(dtype, reinterpret_cast<const b17::BatchInst*>(d),
+                                   reinterpret_cast<const int2*>(d + off_items), nitems, b->d_ctl,
+                                   nctl, threads, lds, b->spin_limit, st, &grid);
+  if (e != hipSuccess) return b17_err(BRI17_ERR_HIP, hipGetErrorString(e));
+  if (hipEventRecord(b->done, st) != hipSuccess)
+    return b17_err(BRI17_ERR_HIP, "hipEventRecord(batch) failed");
+  b->pending = true;
+  b->last_nitems = nitems;
+  b->last_grid = grid;
+  b->last_threads = threads;
+  b->last_lds = lds;
+  return BRI17_OK;
+}
+
+int bri17_batch_wait(bri17_batch* b, int* dependency_timeouts) {
+  if (!b) return b17_err(BRI17_ERR_INVALID, "null batch");
+  std::lock_guard<std::mutex> g(b->mtx);
+  int rc = wait_pending(b);
+  if (rc) return rc;
+  if (dependency_timeouts) {
+    *dependency_timeouts = 0;
+    if (b->d_ctl && hipMemcpy(dependency_timeouts, b->d_ctl + 1, sizeof(int),
+                              hipMemcpyDeviceToHost) != hipSuccess)
+      return b17_err(BRI17_ERR_HIP, "hipMemcpy(batch status) failed");
+  }
+  return BRI17_OK;
+}
+
+int bri17_batch_info(const bri17_batch* b, int* work_items, int* workgroups, int* threads,
+                     size_t* lds_bytes) {
+  if (!b) return b17_err(BRI17_ERR_INVALID, "null batch");
+  if (work_items) *work_items = b->last_nitems;
+  if (workgroups) *workgroups = b->last_grid;
+  if (threads) *threads = b->last_threads;
+  if (lds_bytes) *lds_bytes = b->last_lds;
+  return BRI17_OK;
+}
+
+}  // extern "C"

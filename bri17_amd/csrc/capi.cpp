@@ -12,35 +12,26 @@
 #include <new>
 #include <vector>
 
+#include "capi_internal.hpp"
 #include "plan.hpp"
 
 using b17::Plan;
 using b17::Status;
 
-struct bri17_operator {
-  std::shared_ptr<Plan> plan;
-  double mu = 1.0, nu = 0.0;
-  void* ws = nullptr;         // half-spectrum workspace (device)
-  void* host_in = nullptr;    // staging buffers of the *_host entry points
-  void* host_out = nullptr;
-  std::mutex mtx;             // serialises use of the owned workspace
-  // per-kernel timing (bri17_operator_profile)
-  bool profiling = false;
-  std::vector<hipEvent_t> events;  // 4 per profiled call
-  size_t ev_used = 0;
-};
-
-namespace {
-
-int finish(const Status& s) {
+int b17_finish(const Status& s) {
   if (!s.ok()) b17::set_error(s.msg);
   return s.code;
 }
 
-int err(int code, const char* msg) {
+int b17_err(int code, const char* msg) {
   b17::set_error(msg);
   return code;
 }
+
+namespace {
+
+int finish(const Status& s) { return b17_finish(s); }
+int err(int code, const char* msg) { return b17_err(code, msg); }
 
 bool check_grid(int dim, const int* shape) {
   if (dim != 2 && dim != 3) return false;
@@ -212,21 +203,12 @@ int eigenstress(int dim, const int* shape, const T* L, double mu, double nu, con
   return BRI17_OK;
 }
 
-int ensure_ws(bri17_operator* op) {
-  if (op->ws) return BRI17_OK;
-  hipError_t e = hipMalloc(&op->ws, op->plan->ws_bytes);
-  if (e != hipSuccess) {
-    op->ws = nullptr;
-    return err(BRI17_ERR_NOMEM, "hipMalloc(workspace) failed");
-  }
-  return BRI17_OK;
-}
 
 int run(bri17_operator* op, int mode, const void* in, void* out, void* stream) {
   if (!op || !in || !out) return err(BRI17_ERR_INVALID, "null argument");
   if (in == out) return err(BRI17_ERR_INVALID, "input and output must not alias");
   std::lock_guard<std::mutex> g(op->mtx);
-  int rc = ensure_ws(op);
+  int rc = b17_ensure_ws(op);
   if (rc) return rc;
   hipEvent_t* ev = nullptr;
   if (op->profiling) {
@@ -264,6 +246,16 @@ int run_host(bri17_operator* op, int mode, const void* in, void* out) {
 }
 
 }  // namespace
+
+int b17_ensure_ws(bri17_operator* op) {
+  if (op->ws) return BRI17_OK;
+  hipError_t e = hipMalloc(&op->ws, op->plan->ws_bytes);
+  if (e != hipSuccess) {
+    op->ws = nullptr;
+    return err(BRI17_ERR_NOMEM, "hipMalloc(workspace) failed");
+  }
+  return BRI17_OK;
+}
 
 extern "C" {
 

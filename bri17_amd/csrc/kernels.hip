@@ -10,310 +10,29 @@
 // spectrum along the last axis is formed (R2C / C2R), which is exact.
 #include "fft_core.hpp"
 #include "kernels.hpp"
-#include "modal.hpp"
+#include "kernels_generic.hpp"
 
 namespace b17 {
 
-constexpr int kMaxE = 16;  // max complex elements per thread per FFT pass
-// global loads a thread keeps in flight when staging (fewer for fp64: VGPRs)
-template <typename T>
-constexpr int kLoads = sizeof(T) == 4 ? 16 : 8;
 
-// Copy `tot` elements with up to LB loads in flight per thread: all loads of
-// a batch are issued before any of their values is consumed, so a staging
-// loop is bound by bandwidth instead of one memory latency per element.
-template <int LB, typename V, typename Get, typename Put>
-__device__ __forceinline__ void staged_copy(int tot, int tid, int nthr, Get get, Put put) {
-  for (int e0 = tid; e0 < tot; e0 += LB * nthr) {
-    V v[LB];
-#pragma unroll
-    for (int i = 0; i < LB; i++) {
-      const int e = e0 + i * nthr;
-      if (e < tot) v[i] = get(e);
-    }
-#pragma unroll
-    for (int i = 0; i < LB; i++) {
-      const int e = e0 + i * nthr;
-      if (e < tot) put(e, v[i]);
-    }
-  }
-}
-
-template <typename T>
-__device__ __forceinline__ cpx<T> post_r2c(cpx<T> A, cpx<T> B, cpx<T> w) {
-  // X[k] = (A + conj B)/2 - i w (A - conj B)/2, A = Z[k], B = Z[m-k]
-  cpx<T> e = {T(0.5) * (A.x + B.x), T(0.5) * (A.y - B.y)};
-  cpx<T> d = {T(0.5) * (A.x - B.x), T(0.5) * (A.y + B.y)};
-  cpx<T> o = {d.y, -d.x};  // -i d
-  return e + cmul(w, o);
-}
-
-template <typename T>
-__device__ __forceinline__ cpx<T> pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
-  // Z[k] = (A + conj B) + i conj(w) (A - conj B), A = X[k], B = X[m-k]
-  cpx<T> e = {A.x + B.x, A.y - B.y};
-  cpx<T> d = {A.x - B.x, A.y + B.y};
-  cpx<T> o = cmulc(d, w);
-  return {e.x - o.y, e.y + o.x};
-}
-
-// ------------------------------------------------------------------ plane fwd
-
+// Plain launches, one workgroup per plane block / pencil tile.
 template <typename T>
 __global__ void __launch_bounds__(1024)
     k_plane_fwd(const T* __restrict__ u, cpx<T>* __restrict__ ws, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-  const cpx<T>* twr = reinterpret_cast<const cpx<T>*>(pp.tw_row);
-  const cpx<T>* twc = reinterpret_cast<const cpx<T>*>(pp.tw_col);
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int plane = blockIdx.x / pp.p;
-  const int p = blockIdx.x - plane * pp.p;
-  int nrows = pp.q;
-  if (!pp.has_col) {
-    const int rem = pp.nrows - plane * pp.n1;
-    nrows = rem < pp.q ? rem : pp.q;
-  }
-  const int64_t row0 = (int64_t)plane * pp.n1 + p;
-  const int pst = pp.p;
-
-  // 1. rows -> LDS (even rows are read as m complex pairs)
-  {
-    const int m = pp.m, tot = nrows * m, rsl = pp.rsl;
-    auto put = [&](int e, cpx<T> v) {
-      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      lds[qq * rsl + (e - qq * m)] = v;
-    };
-    if (pp.even) {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
-        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        return reinterpret_cast<const cpx<T>*>(u + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m];
-      }, put);
-    } else {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
-        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        return cpx<T>{u[(row0 + (int64_t)qq * pst) * pp.nl + (e - qq * m)], T(0)};
-      }, put);
-    }
-  }
-  __syncthreads();
-
-  // 2. row FFTs, consecutive threads on consecutive elements of a row
-  {
-    const int tpt = pp.row.tpt, S = nthr / tpt;
-    const int sl = tid / tpt, lane = tid - sl * tpt;
-    for (int r0 = 0; r0 < nrows; r0 += S) {
-      const int s = r0 + sl;
-      const bool act = sl < S && s < nrows;
-      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, lane, act);
-    }
-  }
-
-  // 3. split the packed transform into the nl/2+1 spectrum entries
-  if (pp.even) {
-    const int m = pp.m, h = m / 2 + 1, tot = nrows * h;
-    for (int e = tid; e < tot; e += nthr) {
-      const int qq = (int)fdiv((uint32_t)e, pp.hdiv);
-      const int k = e - qq * h;
-      cpx<T>* row = lds + qq * pp.rsl;
-      if (k == 0) {
-        const cpx<T> z = row[0];
-        row[0] = {z.x + z.y, T(0)};
-        row[m] = {z.x - z.y, T(0)};
-      } else {
-        const int k2 = m - k;
-        const cpx<T> A = row[k], B = row[k2];
-        row[k] = post_r2c(A, B, twr[k]);
-        if (k2 != k) row[k2] = post_r2c(B, A, twr[k2]);
-      }
-    }
-    __syncthreads();
-  }
-
-  // 4. 3D: Q-point stage of the middle axis on every column, then the split
-  //    twiddle w_N1^(p ka) is applied on the way out
-  if (pp.has_col) {
-    const int tpt = pp.col.tpt, S = nthr / tpt;
-    const int sl = tid % S, lane = tid / S;
-    const bool inr = tid < S * tpt;
-    for (int c0 = 0; c0 < pp.nlh; c0 += S) {
-      const int s = c0 + sl;
-      const bool act = inr && s < pp.nlh;
-      lds_fft<T, false, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, lane, act);
-    }
-  }
-
-  // 5. store
-  {
-    const int tot = nrows * pp.nlh, nlh = pp.nlh;
-    cpx<T>* dst = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
-    staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
-      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      cpx<T> v = lds[ka * pp.rsl + (e - ka * nlh)];
-      if (p > 0) v = cmul(v, twc[p * ka]);
-      return v;
-    }, [&](int e, cpx<T> v) {
-      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      dst[(int64_t)ka * pp.rs + (e - ka * nlh)] = v;
-    });
-  }
+  plane_fwd_body<T>(u, ws, pp, blockIdx.x, reinterpret_cast<cpx<T>*>(smem));
 }
-
-// ------------------------------------------------------------------ plane inv
 
 template <typename T>
 __global__ void __launch_bounds__(1024)
     k_plane_inv(const cpx<T>* __restrict__ ws, T* __restrict__ out, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-  const cpx<T>* twr = reinterpret_cast<const cpx<T>*>(pp.tw_row);
-  const cpx<T>* twc = reinterpret_cast<const cpx<T>*>(pp.tw_col);
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int plane = blockIdx.x / pp.p;
-  const int p = blockIdx.x - plane * pp.p;
-  int nrows = pp.q;
-  if (!pp.has_col) {
-    const int rem = pp.nrows - plane * pp.n1;
-    nrows = rem < pp.q ? rem : pp.q;
-  }
-  const int64_t row0 = (int64_t)plane * pp.n1 + p;
-  const int pst = pp.p;
-
-  // 1. load (undo the split twiddle)
-  {
-    const int tot = nrows * pp.nlh, nlh = pp.nlh;
-    const cpx<T>* src = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
-    staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
-      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      return src[(int64_t)ka * pp.rs + (e - ka * nlh)];
-    }, [&](int e, cpx<T> v) {
-      const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      if (p > 0) v = cmulc(v, twc[p * ka]);
-      lds[ka * pp.rsl + (e - ka * nlh)] = v;
-    });
-  }
-  __syncthreads();
-
-  // 2. 3D: inverse Q-point stage of the middle axis
-  if (pp.has_col) {
-    const int tpt = pp.col.tpt, S = nthr / tpt;
-    const int sl = tid % S, lane = tid / S;
-    const bool inr = tid < S * tpt;
-    for (int c0 = 0; c0 < pp.nlh; c0 += S) {
-      const int s = c0 + sl;
-      const bool act = inr && s < pp.nlh;
-      lds_fft<T, true, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, lane, act);
-    }
-  }
-
-  // 3. rebuild the transform the row FFT needs
-  if (pp.even) {
-    const int m = pp.m, h = m / 2 + 1, tot = nrows * h;
-    for (int e = tid; e < tot; e += nthr) {
-      const int qq = (int)fdiv((uint32_t)e, pp.hdiv);
-      const int k = e - qq * h;
-      cpx<T>* row = lds + qq * pp.rsl;
-      if (k == 0) {
-        const cpx<T> A = row[0], B = row[m];
-        const cpx<T> ee = {A.x + B.x, A.y - B.y};
-        const cpx<T> d = {A.x - B.x, A.y + B.y};
-        row[0] = {ee.x - d.y, ee.y + d.x};
-      } else {
-        const int k2 = m - k;
-        const cpx<T> A = row[k], B = row[k2];
-        row[k] = pre_c2r(A, B, twr[k]);
-        if (k2 != k) row[k2] = pre_c2r(B, A, twr[k2]);
-      }
-    }
-  } else {
-    // odd length: Hermitian completion X[n] = conj X[nl - n]
-    const int nl = pp.nl, nh = pp.nlh, ext = nl - nh, tot = nrows * ext;
-    for (int e = tid; e < tot; e += nthr) {
-      const int qq = e / ext;
-      const int n = nh + (e - qq * ext);
-      cpx<T>* row = lds + qq * pp.rsl;
-      row[n] = conj(row[nl - n]);
-    }
-  }
-  __syncthreads();
-
-  // 4. inverse row FFTs
-  {
-    const int tpt = pp.row.tpt, S = nthr / tpt;
-    const int sl = tid / tpt, lane = tid - sl * tpt;
-    for (int r0 = 0; r0 < nrows; r0 += S) {
-      const int s = r0 + sl;
-      const bool act = sl < S && s < nrows;
-      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, lane, act);
-    }
-  }
-
-  // 5. store real rows
-  {
-    const int m = pp.m, tot = nrows * m, rsl = pp.rsl;
-    auto get = [&](int e) {
-      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      return lds[qq * rsl + (e - qq * m)];
-    };
-    if (pp.even) {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, get, [&](int e, cpx<T> v) {
-        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        reinterpret_cast<cpx<T>*>(out + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m] = v;
-      });
-    } else {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, get, [&](int e, cpx<T> v) {
-        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        out[(row0 + (int64_t)qq * pst) * pp.nl + (e - qq * m)] = v.x;
-      });
-    }
-  }
-}
-
-// ---------------------------------------------------------------------- pencil
-
-template <typename T, int R, bool INV>
-__device__ __forceinline__ void pdft_r(cpx<T>* lds, int W, FastDiv wdiv, int items,
-                                       int tid, int nthr) {
-  for (int e = tid; e < items; e += nthr) {
-    const int outer = (int)fdiv((uint32_t)e, wdiv);
-    const int w = e - outer * W;
-    cpx<T>* b = lds + outer * R * W + w;
-    cpx<T> v[R];
-#pragma unroll
-    for (int k = 0; k < R; k++) v[k] = b[k * W];
-    bfly<R, INV>(v);
-#pragma unroll
-    for (int k = 0; k < R; k++) b[k * W] = v[k];
-  }
-}
-
-// P-point DFT over the split index p for every (c, i0, w) of the tile.
-template <typename T, bool INV>
-__device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
-                                     int nouter, int tid, int nthr) {
-  const int items = nouter * W;
-  switch (P) {
-    case 2: pdft_r<T, 2, INV>(lds, W, wdiv, items, tid, nthr); break;
-    case 3: pdft_r<T, 3, INV>(lds, W, wdiv, items, tid, nthr); break;
-    case 4: pdft_r<T, 4, INV>(lds, W, wdiv, items, tid, nthr); break;
-    case 5: pdft_r<T, 5, INV>(lds, W, wdiv, items, tid, nthr); break;
-    case 7: pdft_r<T, 7, INV>(lds, W, wdiv, items, tid, nthr); break;
-    case 8: pdft_r<T, 8, INV>(lds, W, wdiv, items, tid, nthr); break;
-    case 16: pdft_r<T, 16, INV>(lds, W, wdiv, items, tid, nthr); break;
-    default: break;
-  }
-  __syncthreads();
+  plane_inv_body<T>(ws, out, pp, blockIdx.x, reinterpret_cast<cpx<T>*>(smem));
 }
 
 template <typename T, int DIM, int MODE>
 __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const PencilParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-  const cpx<T>* tw0 = reinterpret_cast<const cpx<T>*>(pp.tw0);
-  const ModalCoef<T>* coef[3] = {reinterpret_cast<const ModalCoef<T>*>(pp.coef[0]),
-                                 reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]),
-                                 reinterpret_cast<const ModalCoef<T>*>(pp.coef[2])};
-  const int tid = threadIdx.x, nthr = blockDim.x;
   int t = blockIdx.x;
   if (pp.xcd_remap) {
     // consecutive tiles share L2 lines: keep them on one XCD (blocks are
@@ -322,73 +41,7 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
     const int x = t % 8, idx = t / 8;
     t = x * per + (x < rem ? x : rem) + idx;
   }
-  const int W = pp.w, P = pp.p, N0 = pp.n0, PW = P * W;
-  const int f0 = t * W;
-  const int wv = pp.f - f0 < W ? pp.f - f0 : W;
-  const int tot = DIM * N0 * PW;
-  const int cst = N0 * PW;  // LDS distance between components
-
-  // load the tile: element (c, i0, p, w) at ((c N0 + i0) P + p) W + w
-  staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
-    const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
-    const int w = e - rest * W;
-    cpx<T> v = {T(0), T(0)};
-    if (w < wv) v = ws[(int64_t)rest * pp.f + f0 + w];
-    return v;
-  }, [&](int e, cpx<T> v) { lds[e] = v; });
-  __syncthreads();
-  if (P > 1) pdft<T, false>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
-
-  const int tpt = pp.ax0.tpt, S = nthr / tpt;
-  const int sl = tid % S, lane = tid / S;
-  const bool inr = tid < S * tpt;
-  const int nseq = DIM * PW;
-  for (int s0 = 0; s0 < nseq; s0 += S) {
-    const int s = s0 + sl;
-    const bool act = inr && s < nseq;
-    const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
-    const int base = act ? g * cst + (s - g * PW) : 0;
-    lds_fft<T, false, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
-  }
-
-  {
-    const T mu = T(pp.mu), sc = T(pp.s);
-    const int items = N0 * PW;
-    for (int e = tid; e < items; e += nthr) {
-      const int k0 = (int)fdiv((uint32_t)e, pp.pwdiv);
-      const int r = e - k0 * PW;
-      const int kb = (int)fdiv((uint32_t)r, pp.wdiv);
-      const int w = r - kb * W;
-      if (w >= wv) continue;
-      const int f = f0 + w;
-      const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
-      const int kl = f - ka * pp.rs;
-      if (kl >= pp.nlh) continue;
-      const int k1 = ka + pp.q * kb;
-      cpx<T> x[DIM];
-#pragma unroll
-      for (int c = 0; c < DIM; c++) x[c] = lds[c * cst + e];
-      modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, x);
-#pragma unroll
-      for (int c = 0; c < DIM; c++) lds[c * cst + e] = x[c];
-    }
-  }
-  __syncthreads();
-
-  for (int s0 = 0; s0 < nseq; s0 += S) {
-    const int s = s0 + sl;
-    const bool act = inr && s < nseq;
-    const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
-    const int base = act ? g * cst + (s - g * PW) : 0;
-    lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
-  }
-  if (P > 1) pdft<T, true>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
-
-  staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) { return lds[e]; }, [&](int e, cpx<T> v) {
-    const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
-    const int w = e - rest * W;
-    if (w < wv) ws[(int64_t)rest * pp.f + f0 + w] = v;
-  });
+  pencil_body<T, DIM, MODE>(ws, pp, t, reinterpret_cast<cpx<T>*>(smem));
 }
 
 // ------------------------------------------------------------------ launchers

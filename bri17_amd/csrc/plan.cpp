@@ -319,8 +319,7 @@ std::string describe_plan(const Plan& pl) {
   return o.str();
 }
 
-Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
-                    void* ws, hipStream_t st, hipEvent_t* ev) {
+PencilParams pencil_for(const Plan& pl, double mu, double nu, int mode) {
   PencilParams pc = pl.pencil;
   pc.mode = mode;
   const double N = (double)pl.nnodes;
@@ -331,6 +330,12 @@ Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* 
   const double s = mu / (1.0 - 2.0 * nu);
   pc.mu = mode == 0 ? mu * pc.scale : mu / pc.scale;
   pc.s = mode == 0 ? s * pc.scale : s / pc.scale;
+  return pc;
+}
+
+Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
+                    void* ws, hipStream_t st, hipEvent_t* ev) {
+  const PencilParams pc = pencil_for(pl, mu, nu, mode);
   hipError_t e = hipSuccess;
   if (pl.use_static) {
     e = launch_static(pl.dtype, pl.shape[0], pl.shape[1], pl.shape[2], mode, in, out, ws, pc,

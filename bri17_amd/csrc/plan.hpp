@@ -71,6 +71,10 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype,
 std::string describe_plan(const Plan& pl);
 
 // Launch the three-kernel pipeline.  mode 0: K u, mode 1: K^-1 f.
+// Pencil parameters of a plan for one call: mode (0 apply, 1 solve) and the
+// material constants with the mode's scale folded in.
+PencilParams pencil_for(const Plan& pl, double mu, double nu, int mode);
+
 // `ev`, if not null, holds 4 events recorded before the first kernel and after
 // each of the three kernels (per-kernel timing).
 Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in,

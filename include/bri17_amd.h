@@ -140,6 +140,34 @@ int bri17_operator_solve(bri17_operator* op, const void* f, void* u, void* strea
 int bri17_operator_apply_host(bri17_operator* op, const void* u, void* Ku);
 int bri17_operator_solve_host(bri17_operator* op, const void* f, void* u);
 
+/* ---------------------------------------------------------------- batches */
+
+/* Many operators -- different grids, dimensions and materials, one dtype --
+ * applied or solved in ONE persistent kernel launch (no reference
+ * counterpart: the reference applies one operator per call on the CPU,
+ * tests/test_bri17.cpp:56-107).  A batch context owns the device work queue;
+ * reuse it across calls. */
+typedef struct bri17_batch bri17_batch;
+typedef struct bri17_batch_item {
+  bri17_operator* op; /* each operator at most once per batch */
+  int mode;           /* 0: K u, 1: Green solve */
+  const void* in;     /* device field, DIM*|N| values */
+  void* out;          /* device field, must not alias in */
+} bri17_batch_item;
+
+int bri17_batch_create(bri17_batch** out);
+int bri17_batch_destroy(bri17_batch* b);
+/* Enqueue the batch on `stream` (asynchronous). */
+int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* stream);
+/* Wait for the last run; *dependency_timeouts != 0 if a work item gave up
+ * waiting for its producers (spin limit reached: results are invalid). */
+int bri17_batch_wait(bri17_batch* b, int* dependency_timeouts);
+/* Bound on the wait loops of the persistent kernel (default 2^24 polls). */
+int bri17_batch_set_spin_limit(bri17_batch* b, int limit);
+/* Geometry of the last run. */
+int bri17_batch_info(const bri17_batch* b, int* work_items, int* workgroups, int* threads,
+                     size_t* lds_bytes);
+
 /* Per-kernel timing (no reference counterpart; measurement support).
  * While enabled, every apply/solve records HIP events on its stream around
  * its three kernels (R2C plane sweep, fused axis-0 pencil, C2R plane sweep);

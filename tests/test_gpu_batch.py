@@ -1,0 +1,115 @@
+"""Batched persistent launch: 64 uneven instances (3D boxes with sides from
+{32,48,64,96,128}, 25% 2D grids with sides 2^a 3^b in [64, 512]), per-instance
+materials, apply and Green solve mixed in one launch.  Each instance must equal
+its solo result bitwise (same runtime-size device code, no atomics on data)
+and the fp64 oracle within 1e-5 relative."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+SIDES3 = (32, 48, 64, 96, 128)
+SIDES2 = (64, 72, 81, 96, 108, 128, 144, 162, 192, 216, 243, 256, 288, 324, 384, 432, 486, 512)
+
+
+def make_batch_spec(seed=4, n=64, frac2d=0.25):
+    rng = np.random.default_rng(seed)
+    n2 = int(round(n * frac2d))
+    spec = []
+    for i in range(n):
+        if i < n2:
+            shape = tuple(int(rng.choice(SIDES2)) for _ in range(2))
+        else:
+            shape = tuple(int(rng.choice(SIDES3)) for _ in range(3))
+        mu = float(rng.uniform(0.5, 2.0))
+        nu = float(rng.uniform(0.0, 0.45))
+        mode = "solve" if rng.uniform() < 0.3 else "apply"
+        spec.append((shape, mu, nu, mode))
+    order = rng.permutation(n)
+    return [spec[i] for i in order]
+
+
+def _inputs(device, spec, seed=9):
+    import torch
+    g = torch.Generator(device=device).manual_seed(seed)
+    xs = []
+    for shape, mu, nu, mode in spec:
+        x = torch.rand((len(shape),) + shape, generator=g, device=device, dtype=torch.float32) * 2 - 1
+        if mode == "solve":
+            x -= x.mean(dim=tuple(range(1, len(shape) + 1)), keepdim=True)
+        xs.append(x)
+    return xs
+
+
+def test_batch_of_64_uneven_instances(device, oracle):
+    import torch
+    from bri17_amd import Batch, StiffnessMatrix
+    spec = make_batch_spec()
+    xs = _inputs(device, spec)
+    ops = [StiffnessMatrix(s, None, mu, nu, "float32") for s, mu, nu, _ in spec]
+    outs = [torch.empty_like(x) for x in xs]
+    b = Batch()
+    print("instances:", [s for s, *_ in spec], flush=True)
+    b.run([(op, m, x, o) for op, (s, mu, nu, m), x, o in zip(ops, spec, xs, outs)])
+    assert b.wait() == 0
+    info = b.info()
+    print("batch geometry", info)
+    assert info["work_items"] > 64
+    # solo results through the same runtime-size kernels: bitwise equal
+    worst = 0.0
+    for i, ((s, mu, nu, m), x, o) in enumerate(zip(spec, xs, outs)):
+        solo = StiffnessMatrix(s, None, mu, nu, "float32", tuning=dict(force_generic=1))
+        r = solo.apply(x) if m == "apply" else solo.solve(x)
+        torch.cuda.synchronize()
+        assert torch.equal(r, o), (i, s, m)
+        if i % 4 == 0:
+            x64 = x.double().cpu().numpy()
+            ref = (oracle.apply(s, (1.0,) * len(s), mu, nu, x64, nthreads=NT) if m == "apply"
+                   else oracle.solve(s, (1.0,) * len(s), mu, nu, x64, nthreads=NT))
+            e = rel_err(o.double().cpu().numpy(), ref)
+            worst = max(worst, e)
+            assert e <= 1e-5, (i, s, m, e)
+    print("worst rel err vs fp64 oracle: %.2e" % worst)
+    # a second run of the same batch reproduces the first bitwise
+    outs2 = [torch.empty_like(x) for x in xs]
+    b.run([(op, m, x, o) for op, (s, mu, nu, m), x, o in zip(ops, spec, xs, outs2)])
+    assert b.wait() == 0
+    assert all(torch.equal(a, c) for a, c in zip(outs, outs2))
+
+
+def test_batch_in_fp64_matches_oracle(device, oracle):
+    import torch
+    from bri17_amd import Batch, StiffnessMatrix
+    spec = [((12, 16, 10), 1.1, 0.3, "apply"), ((24, 18), 0.7, 0.2, "solve"),
+            ((3, 4, 5), 5.6, 0.3, "apply"), ((40, 30, 36), 1.9, 0.41, "solve")]
+    xs = [x.double() for x in _inputs(device, spec, seed=3)]
+    ops = [StiffnessMatrix(s, None, mu, nu, "float64") for s, mu, nu, _ in spec]
+    outs = [torch.empty_like(x) for x in xs]
+    b = Batch()
+    b.run([(op, m, x, o) for op, (s, mu, nu, m), x, o in zip(ops, spec, xs, outs)])
+    assert b.wait() == 0
+    for (s, mu, nu, m), x, o in zip(spec, xs, outs):
+        x64 = x.cpu().numpy()
+        ref = (oracle.apply(s, (1.0,) * len(s), mu, nu, x64) if m == "apply"
+               else oracle.solve(s, (1.0,) * len(s), mu, nu, x64))
+        assert rel_err(o.cpu().numpy(), ref) <= 1e-12
+
+
+def test_batch_rejects_bad_items(device):
+    import torch
+    from bri17_amd import Batch, StiffnessMatrix, _lib
+    a = StiffnessMatrix((8, 8, 8), None, 1.0, 0.3, "float32")
+    c = StiffnessMatrix((8, 8), None, 1.0, 0.3, "float64")
+    x = torch.zeros((3, 8, 8, 8), device=device)
+    y = torch.zeros_like(x)
+    x2 = torch.zeros((2, 8, 8), device=device, dtype=torch.float64)
+    b = Batch()
+    with pytest.raises(_lib.Bri17Error):
+        b.run([(a, "apply", x, y), (a, "solve", y, x)])  # same operator twice
+    with pytest.raises(_lib.Bri17Error):
+        b.run([(a, "apply", x, y), (c, "apply", x2, x2.clone())])  # mixed dtypes
