@@ -55,6 +55,13 @@ SIGNATURES = {
     "bri17_batch_wait": (_c.c_int, [_vp, _ip]),
     "bri17_batch_set_spin_limit": (_c.c_int, [_vp, _c.c_int]),
     "bri17_batch_info": (_c.c_int, [_vp, _ip, _ip, _ip, _c.POINTER(_c.c_size_t)]),
+    "bri17_graph_create": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _c.POINTER(_vp)]),
+    "bri17_batch_capture": (_c.c_int, [_vp, _vp, _c.c_int, _c.POINTER(_vp)]),
+    "bri17_graph_launch": (_c.c_int, [_vp, _vp]),
+    "bri17_graph_destroy": (_c.c_int, [_vp]),
+    "bri17_autotune": (_c.c_int, [_c.c_int, _ip, _dp, _c.c_int, _c.c_int, _vp, _dp, _ip]),
+    "bri17_tuning_cache_set": (_c.c_int, [_c.c_int, _ip, _c.c_int, _vp]),
+    "bri17_tuning_cache_clear": (_c.c_int, []),
     "bri17_operator_profile": (_c.c_int, [_vp, _c.c_int]),
     "bri17_operator_profile_read": (_c.c_int, [_vp, _ip, _dp]),
 }

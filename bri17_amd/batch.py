@@ -16,7 +16,7 @@ class Batch:
         _lib.check(_lib.lib().bri17_batch_create(ctypes.byref(h)))
         self._h = h
 
-    def run(self, items, stream=None):
+    def _items(self, items):
         import torch
         arr = (_lib.BatchItem * len(items))()
         dev = None
@@ -32,11 +32,26 @@ class Batch:
             arr[i].mode = MODES[mode] if isinstance(mode, str) else int(mode)
             arr[i].inp = x.data_ptr()
             arr[i].out = out.data_ptr()
+        return arr, dev
+
+    def run(self, items, stream=None):
+        import torch
+        arr, dev = self._items(items)
         if stream is None:
             stream = torch.cuda.current_stream(dev).cuda_stream
         with torch.cuda.device(dev):
             _lib.check(_lib.lib().bri17_batch_run(self._h, ctypes.byref(arr), len(items),
                                                   ctypes.c_void_p(stream)))
+
+    def capture(self, items):
+        """HIP graph replaying this batch (runs it once eagerly first)."""
+        import torch
+        from .operator import Graph
+        arr, dev = self._items(items)
+        h = ctypes.c_void_p()
+        with torch.cuda.device(dev):
+            _lib.check(_lib.lib().bri17_batch_capture(self._h, ctypes.byref(arr), len(items), ctypes.byref(h)))
+        return Graph(h, keep=(self, items))
 
     def wait(self):
         """Block until the last run finished; returns the dependency-timeout flag."""

@@ -23,8 +23,8 @@ struct bri17_batch {
   int* d_ctl = nullptr;
   int nctl = 0;
   int spin_limit = 1 << 24;
-  int last_nitems = 0, last_grid = 0, last_threads = 0;
-  size_t last_lds = 0;
+  int last_nitems = 0, last_grid = 0, last_threads = 0, last_dtype = 0;
+  size_t last_lds = 0, last_off_items = 0;
   std::mutex mtx;
 };
 
@@ -160,8 +160,28 @@ int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* 
   b->last_grid = grid;
   b->last_threads = threads;
   b->last_lds = lds;
+  b->last_dtype = dtype;
+  b->last_off_items = off_items;
   return BRI17_OK;
 }
+
+}  // extern "C"
+
+// Launch the last batch again from the tables already on the device
+// (counter reset + persistent kernel only), e.g. inside a graph capture.
+int b17_batch_relaunch(bri17_batch* b, hipStream_t st) {
+  if (!b || !b->d_buf || !b->last_nitems) return b17_err(BRI17_ERR_STATE, "no batch to relaunch");
+  char* d = static_cast<char*>(b->d_buf);
+  int grid = 0;
+  hipError_t e = b17::launch_batch(b->last_dtype, reinterpret_cast<const b17::BatchInst*>(d),
+                                   reinterpret_cast<const int2*>(d + b->last_off_items),
+                                   b->last_nitems, b->d_ctl, b->nctl, b->last_threads, b->last_lds,
+                                   b->spin_limit, st, &grid);
+  if (e != hipSuccess) return b17_err(BRI17_ERR_HIP, hipGetErrorString(e));
+  return BRI17_OK;
+}
+
+extern "C" {
 
 int bri17_batch_wait(bri17_batch* b, int* dependency_timeouts) {
   if (!b) return b17_err(BRI17_ERR_INVALID, "null batch");

@@ -328,6 +328,9 @@ int bri17_operator_create_tuned(int dim, const int* shape, const double* L, doub
   if (!out || !shape || !L) return err(BRI17_ERR_INVALID, "null argument");
   *out = nullptr;
   b17::Tuning tun;
+  bri17_tuning cached;
+  if (!tuning && (dim == 2 || dim == 3) && b17::cached_tuning(dim, shape, dtype, &cached))
+    tuning = &cached;
   if (tuning) {
     tun.plane_split = tuning->plane_split;
     tun.pencil_width = tuning->pencil_width;

@@ -28,3 +28,10 @@ int b17_finish(const b17::Status& s);
 int b17_err(int code, const char* msg);
 // allocate the operator's workspace on first use
 int b17_ensure_ws(bri17_operator* op);
+// launch the last batch of a context again (tables already on the device)
+int b17_batch_relaunch(bri17_batch* b, hipStream_t st);
+
+namespace b17 {
+// tuning registered with bri17_tuning_cache_set for a grid, if any
+bool cached_tuning(int dim, const int* shape, int dtype, bri17_tuning* t);
+}  // namespace b17

@@ -125,6 +125,22 @@ class StiffnessMatrix:
                           ctypes.c_void_p(stream)))
         return out
 
+    def capture(self, x, out, mode="apply"):
+        """HIP graph of one apply / solve on fixed device tensors (runs once
+        eagerly, then captures); replay with ``Graph.launch``."""
+        import torch
+        want = getattr(torch, self.dtype)
+        for t in (x, out):
+            if (not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != want
+                    or tuple(t.shape) != self.field_shape or not t.is_contiguous()):
+                raise ValueError("expected contiguous %s device tensors of shape %s" % (self.dtype, self.field_shape))
+        h = ctypes.c_void_p()
+        with torch.cuda.device(x.device):
+            _lib.check(_lib.lib().bri17_graph_create(self._h, 0 if mode == "apply" else 1,
+                                                     ctypes.c_void_p(x.data_ptr()),
+                                                     ctypes.c_void_p(out.data_ptr()), ctypes.byref(h)))
+        return Graph(h, keep=(self, x, out))
+
     def apply(self, u, out=None, stream=None):
         """K u (tests/test_bri17.cpp:56-107)."""
         return self._run(u, out, 0, stream)
@@ -134,3 +150,29 @@ class StiffnessMatrix:
         return self._run(f, out, 1, stream)
 
     __matmul__ = apply
+
+
+class Graph:
+    """An instantiated HIP graph (bri17_graph_*); keeps its operator and
+    buffers alive."""
+
+    def __init__(self, handle, keep=()):
+        self._h = handle
+        self._keep = keep
+
+    def launch(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().bri17_graph_launch(self._h, ctypes.c_void_p(stream)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().bri17_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

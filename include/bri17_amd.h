@@ -168,6 +168,36 @@ int bri17_batch_set_spin_limit(bri17_batch* b, int limit);
 int bri17_batch_info(const bri17_batch* b, int* work_items, int* workgroups, int* threads,
                      size_t* lds_bytes);
 
+/* ----------------------------------------------------------------- graphs */
+
+/* A captured HIP graph of one apply / solve on fixed device buffers, or of
+ * one batch, replayed with a single launch (no reference counterpart).  The
+ * operator, batch context and buffers must outlive the graph; replays reuse
+ * the operator's workspace, so replays of one graph must not overlap. */
+typedef struct bri17_graph bri17_graph;
+/* Runs the call once eagerly, then captures it. */
+int bri17_graph_create(bri17_operator* op, int mode, const void* in, void* out,
+                       bri17_graph** graph);
+/* Runs the batch once eagerly (uploading its work queue), then captures the
+ * queue reset and the persistent kernel. */
+int bri17_batch_capture(bri17_batch* b, const bri17_batch_item* items, int n,
+                        bri17_graph** graph);
+int bri17_graph_launch(bri17_graph* graph, void* stream);
+int bri17_graph_destroy(bri17_graph* graph);
+
+/* ------------------------------------------------------------- autotuner */
+
+/* Time candidate launch shapes of the operator of a grid (fixed-size kernels
+ * and their prefetch variants; runtime-size kernels over the middle-axis
+ * split, pencil tile width and radix orders) with `reps` applies each, and
+ * return the fastest.  Nothing is registered. */
+int bri17_autotune(int dim, const int* shape, const double* L, int dtype, int reps,
+                   bri17_tuning* best, double* best_ms, int* candidates_tried);
+/* Register the tuning that bri17_operator_create uses for a (dim, shape,
+ * dtype); the JSON cache file of the Python package is loaded through it. */
+int bri17_tuning_cache_set(int dim, const int* shape, int dtype, const bri17_tuning* t);
+int bri17_tuning_cache_clear(void);
+
 /* Per-kernel timing (no reference counterpart; measurement support).
  * While enabled, every apply/solve records HIP events on its stream around
  * its three kernels (R2C plane sweep, fused axis-0 pencil, C2R plane sweep);
