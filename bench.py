@@ -96,6 +96,139 @@ def cpu_baseline(f_host, shape, mu, nu, threads):
     return out
 
 
+SIDES3_BATCH = (32, 48, 64, 96, 128)
+SIDES2_BATCH = (64, 72, 81, 96, 108, 128, 144, 162, 192, 216, 243, 256, 288, 324, 384, 432, 486, 512)
+SIDES_SERVE = (32, 36, 48, 54, 64, 72, 81, 96, 108, 128, 144, 162, 192, 216, 243, 256)
+
+
+def batched(dev, steps, warmup, seed=4):
+    """64 instances in one persistent launch, fp32: 48 3D grids with sides
+    from {32,48,64,96,128}, 16 2D grids with sides 2^a 3^b in [64,512],
+    mu ~ U[0.5,2], nu ~ U[0,0.45], 30% Green solves, u ~ U[-1,1]."""
+    import numpy as np
+    import torch
+    from bri17_amd import Batch, StiffnessMatrix
+    rng = np.random.default_rng(seed)
+    spec = []
+    for i in range(64):
+        shape = tuple(int(rng.choice(SIDES2_BATCH if i < 16 else SIDES3_BATCH)) for _ in range(2 if i < 16 else 3))
+        spec.append((shape, float(rng.uniform(0.5, 2.0)), float(rng.uniform(0.0, 0.45)),
+                     "solve" if rng.uniform() < 0.3 else "apply"))
+    g = torch.Generator(device=dev).manual_seed(seed)
+    ops = [StiffnessMatrix(sh, None, mu, nu, "float32") for sh, mu, nu, _ in spec]
+    xs = [torch.rand(op.field_shape, generator=g, device=dev) * 2 - 1 for op in ops]
+    outs = [torch.empty_like(x) for x in xs]
+    items = [(op, m, x, o) for op, (_, _, _, m), x, o in zip(ops, spec, xs, outs)]
+    b = Batch()
+    for _ in range(warmup):
+        b.run(items)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run(items)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    assert b.wait() == 0
+    nodes = sum(int(np.prod(sh)) for sh, *_ in spec)
+    b_alg = sum(2 * len(sh) * int(np.prod(sh)) * 4 for sh, *_ in spec)
+    # the same 64 instances one operator call after another, for comparison
+    def seq():
+        for op, m, x, o in items:
+            (op.apply if m == "apply" else op.solve)(x, out=o)
+    for _ in range(2):
+        seq()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        seq()
+    torch.cuda.synchronize()
+    t_seq = (time.perf_counter() - t1) / steps
+    return {"instances": 64, "nodes": nodes, "ms_per_launch": 1e3 * t, "instances_per_s": 64 / t,
+            "achieved_GBs": b_alg / t / 1e9, "algorithmic_bytes": b_alg,
+            "ms_sequential_calls": 1e3 * t_seq, "speedup_vs_sequential": t_seq / t, "launch": b.info()}
+
+
+def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
+    """Poisson arrivals of fp32 3D requests, side log-uniform in [32,256]
+    rounded (in log space) to 2^a 3^b, 70% K.u / 30% Green solve, mu ~
+    U[0.5,2], nu ~ U[0,0.45].  The arrival rate is `load` times the capacity
+    measured by a closed-loop burst.  Inputs stay resident in HBM (one per
+    side, read-only); outputs come from a ring per side whose slots are
+    reused only after their previous request finished."""
+    import numpy as np
+    import torch
+    from bri17_amd import Server
+    from bri17_amd.server import poisson_arrivals, sleep_until
+    rng = np.random.default_rng(seed)
+    logs = np.log(np.array(SIDES_SERVE, dtype=float))
+
+    def draw(n):
+        x = rng.uniform(np.log(32), np.log(256), size=n)
+        side = [SIDES_SERVE[int(np.argmin(np.abs(logs - v)))] for v in x]
+        mode = ["solve" if u < 0.3 else "apply" for u in rng.uniform(size=n)]
+        return list(zip(side, mode, rng.uniform(0.5, 2.0, size=n), rng.uniform(0.0, 0.45, size=n)))
+
+    g = torch.Generator(device=dev).manual_seed(seed)
+    inputs = {n: torch.rand((3, n, n, n), generator=g, device=dev) * 2 - 1 for n in SIDES_SERVE}
+    ring = 16
+    outs = {n: [torch.empty_like(inputs[n]) for _ in range(ring)] for n in SIDES_SERVE}
+    torch.cuda.synchronize()
+    srv = Server()
+    holder = {n: [None] * ring for n in SIDES_SERVE}
+    count = {n: 0 for n in SIDES_SERVE}
+
+    def submit(req):
+        n, mode, mu, nu = req
+        k = count[n] % ring
+        count[n] += 1
+        prev = holder[n][k]
+        if prev is not None:
+            prev.wait()
+        r = srv.submit(inputs[n], outs[n][k], mode, float(mu), float(nu))
+        holder[n][k] = r
+        return r
+
+    # warm-up: every side and mode once (plans, pools)
+    for n in SIDES_SERVE:
+        for m in ("apply", "solve"):
+            submit((n, m, 1.0, 0.3))
+    srv.drain()
+    # capacity: closed-loop burst
+    burst = draw(n_capacity)
+    t0 = time.perf_counter()
+    reqs = [submit(q) for q in burst]
+    for r in reqs:
+        r.wait()
+    cap = n_capacity / (time.perf_counter() - t0)
+    # open loop: Poisson arrivals at load * capacity
+    rate = load * cap
+    arr = poisson_arrivals(rate, n_requests, seed=seed + 1)
+    mix = draw(n_requests)
+    st0 = srv.stats()
+    reqs = []
+    t0 = time.perf_counter()
+    for a, q in zip(arr, mix):
+        sleep_until(t0 + a)
+        reqs.append(submit(q))
+    res = [r.wait() for r in reqs]
+    t_end = time.perf_counter()
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["latency_s"] for r in res]) * 1e3
+    ok = sum(r["state"] == "done" for r in res)
+    waves = st["waves"] - st0["waves"]
+    nodes = sum(n ** 3 for n, *_ in mix)
+    return {"requests": n_requests, "completed": ok, "arrival_rate_hz": rate, "load": load,
+            "capacity_rps": cap, "throughput_rps": n_requests / (t_end - t0),
+            "throughput_Mnodes_per_s": nodes / (t_end - t0) / 1e6,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "waves": waves, "mean_wave_size": n_requests / max(waves, 1),
+            "peak_waves_in_flight": st["peak_waves_in_flight"], "hbm_budget": st["hbm_budget"],
+            "hbm_peak": st["hbm_peak"], "plans_resident": st["plans_resident"],
+            "mix": "3D fp32, side log-uniform [32,256] -> 2^a 3^b, 70% apply / 30% solve"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +237,9 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain fp64 path")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched launch")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-requests", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -197,6 +333,10 @@ def main():
         "green_solves_per_s": K / t_solve,
     }
     line.update(result)
+    if not args.no_batch:
+        line["batched_64"] = batched(dev, max(5, K // 10), 3)
+    if not args.no_serve:
+        line["serving"] = serving(dev, n_requests=args.serve_requests)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(f.double().cpu().numpy(), shape, mu, nu, threads)

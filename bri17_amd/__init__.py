@@ -6,7 +6,8 @@ ABI of include/bri17_amd.h).  Importing the package does not touch the GPU.
 from . import _lib
 from .operator import StiffnessMatrix
 from .batch import Batch
+from .server import Server
 from . import pybri17
 
-__all__ = ["StiffnessMatrix", "Batch", "pybri17"]
+__all__ = ["StiffnessMatrix", "Batch", "Server", "pybri17"]
 __version__ = "0.1.0"

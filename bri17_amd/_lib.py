@@ -64,7 +64,22 @@ SIGNATURES = {
     "bri17_tuning_cache_clear": (_c.c_int, []),
     "bri17_operator_profile": (_c.c_int, [_vp, _c.c_int]),
     "bri17_operator_profile_read": (_c.c_int, [_vp, _ip, _dp]),
+    "bri17_server_create": (_c.c_int, [_vp, _c.POINTER(_vp)]),
+    "bri17_server_destroy": (_c.c_int, [_vp]),
+    "bri17_server_submit": (_c.c_int, [_vp, _vp, _c.POINTER(_c.c_int64)]),
+    "bri17_server_submit_many": (_c.c_int, [_vp, _vp, _c.c_int, _c.POINTER(_c.c_int64)]),
+    "bri17_server_poll": (_c.c_int, [_vp, _c.c_int64, _vp]),
+    "bri17_server_wait": (_c.c_int, [_vp, _c.c_int64, _c.c_double, _vp]),
+    "bri17_server_cancel": (_c.c_int, [_vp, _c.c_int64]),
+    "bri17_server_drain": (_c.c_int, [_vp]),
+    "bri17_server_get_stats": (_c.c_int, [_vp, _vp]),
+    "bri17_server_trim": (_c.c_int, [_vp]),
+    "bri17_server_set_spin_limit": (_c.c_int, [_vp, _c.c_int]),
 }
+
+BRI17_MEM_DEVICE = 0
+BRI17_MEM_HOST = 1
+REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
 
 
 
@@ -92,6 +107,35 @@ class Tuning(ctypes.Structure):
             for i, v in enumerate(rs or ()):
                 t.radices[r][i] = v
         return t
+
+
+class ServerConfig(ctypes.Structure):
+    """bri17_server_config of include/bri17_amd.h."""
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int),
+                ("max_waves", ctypes.c_int), ("solo_nodes", ctypes.c_int64),
+                ("spin_limit", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    """bri17_request of include/bri17_amd.h."""
+    _fields_ = [("dim", ctypes.c_int), ("shape", ctypes.c_int * 3), ("L", ctypes.c_double * 3),
+                ("mu", ctypes.c_double), ("nu", ctypes.c_double), ("mode", ctypes.c_int),
+                ("dtype", ctypes.c_int), ("memory", ctypes.c_int), ("inp", ctypes.c_void_p),
+                ("out", ctypes.c_void_p)]
+
+
+class RequestResult(ctypes.Structure):
+    """bri17_request_result of include/bri17_amd.h."""
+    _fields_ = [("state", ctypes.c_int), ("error", ctypes.c_int), ("queued_s", ctypes.c_double),
+                ("latency_s", ctypes.c_double), ("wave", ctypes.c_int64), ("wave_size", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    """bri17_server_stats of include/bri17_amd.h."""
+    _fields_ = [(n, ctypes.c_int64) for n in ("submitted", "completed", "failed", "rejected", "cancelled",
+                                              "waves", "batched_waves", "solo_waves", "queued", "running")] + \
+               [(n, ctypes.c_size_t) for n in ("hbm_budget", "hbm_admitted", "hbm_cached", "hbm_peak")] + \
+               [("plans_resident", ctypes.c_int64), ("peak_waves_in_flight", ctypes.c_int64)]
 
 
 _lib = None

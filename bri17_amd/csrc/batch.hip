@@ -10,9 +10,10 @@
 // instance are done, an inverse plane item until all its pencil tiles are.
 // A waiting workgroup only waits on items already taken by running
 // workgroups, so the queue always drains.  Completion counters are published
-// with device-scope release and read with acquire, which makes the producers'
-// workspace writes visible across XCDs.  The wait is bounded: past the spin
-// limit the kernel records an error and moves on.
+// with one device-scope release per item and polled with relaxed loads
+// followed by one acquire, which makes the producers' workspace writes
+// visible across XCDs without flushing caches on every poll.  The wait is
+// bounded: past the spin limit the kernel records an error and moves on.
 #include "batch.hpp"
 #include "kernels_generic.hpp"
 
@@ -24,18 +25,24 @@ __global__ void __launch_bounds__(1024)
             int* __restrict__ ctl, int spin_limit) {
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* lds = reinterpret_cast<cpx<T>*>(smem);
-  __shared__ int s_item;
+  __shared__ int s_item[2];
   const int tid = threadIdx.x;
   int* head = ctl;
   int* err = ctl + 1;
   int* done = ctl + 2;
+  if (tid == 0) s_item[0] = atomicAdd(head, 1);
+  __syncthreads();
+  // the item index is workgroup-uniform: keep it (and everything addressed
+  // through it) in scalar registers
+  int cur = __builtin_amdgcn_readfirstlane(s_item[0]);
+  int slot = 0;
   for (;;) {
-    if (tid == 0) s_item = atomicAdd(head, 1);
-    __syncthreads();
-    const int it = s_item;
-    __syncthreads();
-    if (it >= nitems) break;
-    const int2 w = items[it];
+    if (cur >= nitems) break;
+    // claim the next item while this one runs (items are claimed in queue
+    // order, so the lowest unfinished item is always some workgroup's
+    // current one and its dependencies are finished)
+    if (tid == 0) s_item[slot ^ 1] = atomicAdd(head, 1);
+    const int2 w = items[cur];
     const int inst = w.x & 0xffff;
     const int type = w.x >> 16;
     const int idx = w.y;
@@ -45,24 +52,26 @@ __global__ void __launch_bounds__(1024)
         const int need = type == 1 ? I->na : I->nb;
         int* cnt = done + 2 * inst + (type - 1);
         int n = 0;
-        while (__hip_atomic_load(cnt, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < need) {
+        // poll with relaxed device-scope loads (they read past this XCD's
+        // L2 without invalidating it), then acquire once: the waves of a
+        // workgroup share this CU's caches
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
           if (++n > spin_limit) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             break;
           }
           __builtin_amdgcn_s_sleep(4);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       }
       __syncthreads();
     }
     if (type == 0) {
-      plane_fwd_body<T>(reinterpret_cast<const T*>(I->in), reinterpret_cast<cpx<T>*>(I->ws), I->pp,
-                        idx, lds);
+      plane_fwd_body<T>((gptr<const T>)I->in, (gptr<cpx<T>>)I->ws, I->pp, idx, lds);
     } else if (type == 2) {
-      plane_inv_body<T>(reinterpret_cast<const cpx<T>*>(I->ws), reinterpret_cast<T*>(I->out), I->pp,
-                        idx, lds);
+      plane_inv_body<T>((gptr<const cpx<T>>)I->ws, (gptr<T>)I->out, I->pp, idx, lds);
     } else {
-      cpx<T>* ws = reinterpret_cast<cpx<T>*>(I->ws);
+      gptr<cpx<T>> ws = (gptr<cpx<T>>)I->ws;
       switch (I->pc.dim * 2 + I->pc.mode) {
         case 4: pencil_body<T, 2, 0>(ws, I->pc, idx, lds); break;
         case 5: pencil_body<T, 2, 1>(ws, I->pc, idx, lds); break;
@@ -70,11 +79,15 @@ __global__ void __launch_bounds__(1024)
         default: pencil_body<T, 3, 1>(ws, I->pc, idx, lds); break;
       }
     }
-    // every wave waits for its own stores, then one release publishes them
-    __threadfence();
+    // every wave waits until its own stores are complete (in its XCD's L2),
+    // then one device-scope release (L2 write-back) publishes the workgroup's
+    // results to the other XCDs
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0 && type < 2)
       __hip_atomic_fetch_add(done + 2 * inst + type, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    slot ^= 1;
+    cur = __builtin_amdgcn_readfirstlane(s_item[slot]);
   }
 }
 

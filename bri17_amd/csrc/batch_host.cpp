@@ -90,32 +90,61 @@ int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* 
       return b17_err(BRI17_ERR_INVALID,
                      "an operator appears twice in one batch (its workspace would be shared)");
   }
-  int rc = wait_pending(b);
-  if (rc) return rc;
-
-  std::vector<b17::BatchInst> insts((size_t)n);
-  std::vector<int2> work;
-  int threads = 64;
+  std::vector<const b17::Plan*> plans((size_t)n);
+  for (int i = 0; i < n; i++) plans[i] = items[i].op->plan.get();
+  int threads = 0;
   size_t lds = 0;
+  b17_batch_shape(plans, threads, lds);
+  std::vector<b17::BatchInst> insts((size_t)n);
   for (int i = 0; i < n; i++) {
     bri17_operator* op = items[i].op;
     std::lock_guard<std::mutex> og(op->mtx);
-    rc = b17_ensure_ws(op);
+    int rc = b17_ensure_ws(op);
     if (rc) return rc;
-    const b17::Plan& pl = *op->plan;
-    b17::BatchInst& I = insts[i];
-    std::memset(&I, 0, sizeof(I));
-    I.pp = pl.plane;
-    I.pc = b17::pencil_for(pl, op->mu, op->nu, items[i].mode);
-    I.pc.xcd_remap = 0;
-    I.in = items[i].in;
-    I.out = items[i].out;
-    I.ws = op->ws;
-    I.na = pl.plane_blocks;
-    I.nb = pl.pencil.ntiles;
-    threads = std::max({threads, pl.plane_threads, pl.pencil_threads});
-    lds = std::max({lds, pl.plane_lds, pl.pencil_lds});
+    insts[i] = b17_batch_inst(*op->plan, op->mu, op->nu, items[i].mode, items[i].in, items[i].out,
+                              op->ws, lds);
   }
+  return b17_batch_launch(b, dtype, insts, threads, lds, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+// Workgroups of the batch kernel are sized for many small items per CU (four
+// 256-thread workgroups with 39 KiB each), not for the largest solo launch:
+// each plan's items are re-tiled to fit (b17::batch_geometry).
+void b17_batch_shape(const std::vector<const b17::Plan*>& plans, int& threads, size_t& lds) {
+  threads = 256;
+  lds = 39 * 1024;
+  for (const b17::Plan* pl : plans) {
+    size_t l = 0;
+    int t = 0;
+    b17::batch_minimum(*pl, l, t);
+    threads = std::max(threads, t);
+    lds = std::max(lds, l);
+  }
+  threads = std::min(threads, 1024);
+}
+
+b17::BatchInst b17_batch_inst(const b17::Plan& pl, double mu, double nu, int mode, const void* in,
+                              void* out, void* ws, size_t lds) {
+  b17::BatchInst I;
+  std::memset(&I, 0, sizeof(I));
+  I.pc = b17::pencil_for(pl, mu, nu, mode);
+  (void)b17::batch_geometry(pl, lds, I.pp, I.na, I.pc);  // fits: lds >= batch_minimum
+  I.pc.xcd_remap = 0;
+  I.in = in;
+  I.out = out;
+  I.ws = ws;
+  I.nb = I.pc.ntiles;
+  return I;
+}
+
+int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst>& insts,
+                     int threads, size_t lds, hipStream_t st) {
+  const int n = (int)insts.size();
+  int rc = wait_pending(b);
+  if (rc) return rc;
+  std::vector<int2> work;
   // queue: every forward plane block, then every pencil tile, then every
   // inverse plane block -- an item depends only on items before it
   for (int type = 0; type < 3; type++)
@@ -142,7 +171,6 @@ int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* 
   char* h = static_cast<char*>(b->h_buf);
   std::memcpy(h, insts.data(), sizeof(b17::BatchInst) * n);
   std::memcpy(h + off_items, work.data(), sizeof(int2) * work.size());
-  hipStream_t st = (hipStream_t)stream;
   if (hipMemcpyAsync(b->d_buf, b->h_buf, off_ctl, hipMemcpyHostToDevice, st) != hipSuccess)
     return b17_err(BRI17_ERR_HIP, "hipMemcpyAsync(batch tables) failed");
   char* d = static_cast<char*>(b->d_buf);
@@ -165,7 +193,7 @@ int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* 
   return BRI17_OK;
 }
 
-}  // extern "C"
+int* b17_batch_ctl(bri17_batch* b) { return b ? b->d_ctl : nullptr; }
 
 // Launch the last batch again from the tables already on the device
 // (counter reset + persistent kernel only), e.g. inside a graph capture.

@@ -9,7 +9,10 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <map>
 #include <new>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "capi_internal.hpp"
@@ -26,6 +29,50 @@ int b17_finish(const Status& s) {
 int b17_err(int code, const char* msg) {
   b17::set_error(msg);
   return code;
+}
+
+int b17_acquire_plan(int dim, const int* shape, const double* L, int dtype,
+                     const bri17_tuning* tuning, std::shared_ptr<b17::Plan>& out) {
+  if (!shape || !L) return b17_err(BRI17_ERR_INVALID, "null argument");
+  b17::Tuning tun;
+  bri17_tuning cached;
+  if (!tuning && (dim == 2 || dim == 3) && b17::cached_tuning(dim, shape, dtype, &cached))
+    tuning = &cached;
+  if (tuning) {
+    tun.plane_split = tuning->plane_split;
+    tun.pencil_width = tuning->pencil_width;
+    if (tuning->lds_budget_kb > 0) tun.lds_budget = (size_t)tuning->lds_budget_kb * 1024;
+    if (tuning->xcd_remap >= 0) tun.xcd_remap = tuning->xcd_remap;
+    tun.force_generic = tuning->force_generic;
+    if (tuning->plane_prefetch >= 0) tun.plane_prefetch = tuning->plane_prefetch;
+    for (int r = 0; r < 3; r++)
+      for (int i = 0; i < 8 && tuning->radices[r][i] > 0; i++)
+        tun.radices[r].push_back(tuning->radices[r][i]);
+  }
+  // plans (device twiddle / modal tables + launch geometry) are shared by all
+  // live users of the same device, grid, dtype and tuning
+  int device = -1;  // without a device build_plan reports the error below
+  if (hipGetDevice(&device) != hipSuccess) device = -1;
+  std::ostringstream key;
+  key << device << "|" << dim << "|" << dtype;
+  for (int d = 0; d < dim && d < 3; d++) key << "|" << shape[d] << "|" << std::hexfloat << L[d];
+  key << "|" << tun.plane_split << "|" << tun.pencil_width << "|" << tun.lds_budget << "|"
+      << tun.xcd_remap << "|" << tun.force_generic << "|" << tun.plane_prefetch;
+  for (int r = 0; r < 3; r++) {
+    key << "|";
+    for (int x : tun.radices[r]) key << x << ",";
+  }
+  static std::mutex cache_mtx;
+  static std::map<std::string, std::weak_ptr<Plan>> cache;
+  std::lock_guard<std::mutex> g(cache_mtx);
+  auto it = cache.find(key.str());
+  if (it != cache.end()) out = it->second.lock();
+  if (!out) {
+    int rc = b17_finish(b17::build_plan(dim, shape, L, dtype, tun, out));
+    if (rc) return rc;
+    cache[key.str()] = out;
+  }
+  return BRI17_OK;
 }
 
 namespace {
@@ -327,23 +374,8 @@ int bri17_operator_create_tuned(int dim, const int* shape, const double* L, doub
                                 bri17_operator** out) {
   if (!out || !shape || !L) return err(BRI17_ERR_INVALID, "null argument");
   *out = nullptr;
-  b17::Tuning tun;
-  bri17_tuning cached;
-  if (!tuning && (dim == 2 || dim == 3) && b17::cached_tuning(dim, shape, dtype, &cached))
-    tuning = &cached;
-  if (tuning) {
-    tun.plane_split = tuning->plane_split;
-    tun.pencil_width = tuning->pencil_width;
-    if (tuning->lds_budget_kb > 0) tun.lds_budget = (size_t)tuning->lds_budget_kb * 1024;
-    if (tuning->xcd_remap >= 0) tun.xcd_remap = tuning->xcd_remap;
-    tun.force_generic = tuning->force_generic;
-    if (tuning->plane_prefetch >= 0) tun.plane_prefetch = tuning->plane_prefetch;
-    for (int r = 0; r < 3; r++)
-      for (int i = 0; i < 8 && tuning->radices[r][i] > 0; i++)
-        tun.radices[r].push_back(tuning->radices[r][i]);
-  }
   std::shared_ptr<Plan> pl;
-  int rc = finish(b17::build_plan(dim, shape, L, dtype, tun, pl));
+  int rc = b17_acquire_plan(dim, shape, L, dtype, tuning, pl);
   if (rc) return rc;
   auto* op = new (std::nothrow) bri17_operator;
   if (!op) return err(BRI17_ERR_NOMEM, "out of host memory");

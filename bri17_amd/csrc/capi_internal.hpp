@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "bri17_amd.h"
+#include "batch.hpp"
 #include "plan.hpp"
 
 struct bri17_operator {
@@ -26,10 +27,28 @@ struct bri17_operator {
 // status helpers: record the message for bri17_last_error and return code
 int b17_finish(const b17::Status& s);
 int b17_err(int code, const char* msg);
+// shared plan of a grid (built on first use, kept while any user holds it);
+// `tuning` NULL: the tuning registered for the grid, else the defaults
+int b17_acquire_plan(int dim, const int* shape, const double* L, int dtype,
+                     const bri17_tuning* tuning, std::shared_ptr<b17::Plan>& out);
 // allocate the operator's workspace on first use
 int b17_ensure_ws(bri17_operator* op);
 // launch the last batch of a context again (tables already on the device)
 int b17_batch_relaunch(bri17_batch* b, hipStream_t st);
+// workgroup shape of a persistent batched launch over these plans: 256
+// threads and 39 KiB of LDS unless an item needs more
+void b17_batch_shape(const std::vector<const b17::Plan*>& plans, int& threads, size_t& lds);
+// descriptor of one instance of a persistent batched launch whose
+// workgroups have `lds` bytes of LDS (from b17_batch_shape)
+b17::BatchInst b17_batch_inst(const b17::Plan& pl, double mu, double nu, int mode, const void* in,
+                              void* out, void* ws, size_t lds);
+// upload an instance table and launch the persistent kernel on `st` (waits
+// for the context's previous launch first); threads / lds are the maxima
+// over the instances' plans.  The caller serialises use of `b`.
+int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst>& insts,
+                     int threads, size_t lds, hipStream_t st);
+// device address of the control words [head, error, done...] of the last launch
+int* b17_batch_ctl(bri17_batch* b);
 
 namespace b17 {
 // tuning registered with bri17_tuning_cache_set for a grid, if any

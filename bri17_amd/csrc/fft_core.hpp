@@ -22,6 +22,19 @@
 
 namespace b17 {
 
+// Pointer into global memory.  Pointers that a kernel reads from memory (an
+// instance table, the tables of a parameter block) are otherwise generic and
+// their loads and stores compile to flat_* instructions; global_* ones do
+// not wait on LDS traffic.
+// (The host pass of hipcc parses device code too and only sees plain pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using gptr = T*;
+#endif
+
 template <typename T>
 struct cpx {
   T x, y;
@@ -231,8 +244,8 @@ struct SeqLayout {
 
 // One Stockham pass of radix R over the sequences this thread works on.
 // `lane` in [0, tpt) indexes the thread within its sequence.
-template <typename T, int R, bool INV, int MAXB>
-__device__ __forceinline__ void fft_pass(cpx<T>* buf, const cpx<T>* tw,
+template <typename T, int R, bool INV, int MAXB, typename TWP>
+__device__ __forceinline__ void fft_pass(cpx<T>* buf, TWP tw,
                                          const PassDesc& P, int N, int tws,
                                          int base, int es, int lane, int tpt,
                                          bool active) {
@@ -280,8 +293,9 @@ __device__ __forceinline__ void fft_pass(cpx<T>* buf, const cpx<T>* tw,
 // Full FFT of length D.n on the LDS sequence at `base` (stride es).  Every
 // thread of the workgroup must call this (barriers inside); `active` marks
 // threads that own a sequence.
-template <typename T, bool INV, int MAXE>
-__device__ __forceinline__ void lds_fft(cpx<T>* buf, const cpx<T>* tw,
+// `tw` is a pointer to const cpx<T> in any address space.
+template <typename T, bool INV, int MAXE, typename TWP>
+__device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
                                         const FftDesc& D, int base, int es,
                                         int lane, bool active) {
   for (int p = 0; p < D.npass; p++) {

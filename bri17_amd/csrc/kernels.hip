@@ -20,14 +20,16 @@ template <typename T>
 __global__ void __launch_bounds__(1024)
     k_plane_fwd(const T* __restrict__ u, cpx<T>* __restrict__ ws, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  plane_fwd_body<T>(u, ws, pp, blockIdx.x, reinterpret_cast<cpx<T>*>(smem));
+  plane_fwd_body<T>((gptr<const T>)u, (gptr<cpx<T>>)ws, pp, blockIdx.x,
+                    reinterpret_cast<cpx<T>*>(smem));
 }
 
 template <typename T>
 __global__ void __launch_bounds__(1024)
     k_plane_inv(const cpx<T>* __restrict__ ws, T* __restrict__ out, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  plane_inv_body<T>(ws, out, pp, blockIdx.x, reinterpret_cast<cpx<T>*>(smem));
+  plane_inv_body<T>((gptr<const cpx<T>>)ws, (gptr<T>)out, pp, blockIdx.x,
+                    reinterpret_cast<cpx<T>*>(smem));
 }
 
 template <typename T, int DIM, int MODE>
@@ -41,7 +43,7 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
     const int x = t % 8, idx = t / 8;
     t = x * per + (x < rem ? x : rem) + idx;
   }
-  pencil_body<T, DIM, MODE>(ws, pp, t, reinterpret_cast<cpx<T>*>(smem));
+  pencil_body<T, DIM, MODE>((gptr<cpx<T>>)ws, pp, t, reinterpret_cast<cpx<T>*>(smem));
 }
 
 // ------------------------------------------------------------------ launchers

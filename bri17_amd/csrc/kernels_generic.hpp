@@ -58,10 +58,11 @@ __device__ __forceinline__ cpx<T> pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
 // ------------------------------------------------------------------ plane fwd
 
 template <typename T>
-__device__ __forceinline__ void plane_fwd_body(const T* __restrict__ u, cpx<T>* __restrict__ ws,
-                                               const PlaneParams& pp, int blk, cpx<T>* lds) {
-  const cpx<T>* twr = reinterpret_cast<const cpx<T>*>(pp.tw_row);
-  const cpx<T>* twc = reinterpret_cast<const cpx<T>*>(pp.tw_col);
+__device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
+                                               gptr<cpx<T>> __restrict__ ws, const PlaneParams& pp,
+                                               int blk, cpx<T>* lds) {
+  const gptr<const cpx<T>> twr = (gptr<const cpx<T>>)pp.tw_row;
+  const gptr<const cpx<T>> twc = (gptr<const cpx<T>>)pp.tw_col;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int plane = blk / pp.p;
   const int p = blk - plane * pp.p;
@@ -83,7 +84,7 @@ __device__ __forceinline__ void plane_fwd_body(const T* __restrict__ u, cpx<T>* 
     if (pp.even) {
       staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        return reinterpret_cast<const cpx<T>*>(u + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m];
+        return ((gptr<const cpx<T>>)(u + (row0 + (int64_t)qq * pst) * pp.nl))[e - qq * m];
       }, put);
     } else {
       staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
@@ -142,7 +143,7 @@ __device__ __forceinline__ void plane_fwd_body(const T* __restrict__ u, cpx<T>* 
   // 5. store
   {
     const int tot = nrows * pp.nlh, nlh = pp.nlh;
-    cpx<T>* dst = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
+    gptr<cpx<T>> dst = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
     staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
       const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
       cpx<T> v = lds[ka * pp.rsl + (e - ka * nlh)];
@@ -158,10 +159,11 @@ __device__ __forceinline__ void plane_fwd_body(const T* __restrict__ u, cpx<T>* 
 // ------------------------------------------------------------------ plane inv
 
 template <typename T>
-__device__ __forceinline__ void plane_inv_body(const cpx<T>* __restrict__ ws, T* __restrict__ out,
-                                               const PlaneParams& pp, int blk, cpx<T>* lds) {
-  const cpx<T>* twr = reinterpret_cast<const cpx<T>*>(pp.tw_row);
-  const cpx<T>* twc = reinterpret_cast<const cpx<T>*>(pp.tw_col);
+__device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ ws,
+                                               gptr<T> __restrict__ out, const PlaneParams& pp,
+                                               int blk, cpx<T>* lds) {
+  const gptr<const cpx<T>> twr = (gptr<const cpx<T>>)pp.tw_row;
+  const gptr<const cpx<T>> twc = (gptr<const cpx<T>>)pp.tw_col;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int plane = blk / pp.p;
   const int p = blk - plane * pp.p;
@@ -176,7 +178,7 @@ __device__ __forceinline__ void plane_inv_body(const cpx<T>* __restrict__ ws, T*
   // 1. load (undo the split twiddle)
   {
     const int tot = nrows * pp.nlh, nlh = pp.nlh;
-    const cpx<T>* src = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
+    gptr<const cpx<T>> src = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
     staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
       const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
       return src[(int64_t)ka * pp.rs + (e - ka * nlh)];
@@ -300,12 +302,12 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
 }
 
 template <typename T, int DIM, int MODE>
-__device__ __forceinline__ void pencil_body(cpx<T>* __restrict__ ws, const PencilParams& pp, int t,
+__device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const PencilParams& pp, int t,
                                             cpx<T>* lds) {
-  const cpx<T>* tw0 = reinterpret_cast<const cpx<T>*>(pp.tw0);
-  const ModalCoef<T>* coef[3] = {reinterpret_cast<const ModalCoef<T>*>(pp.coef[0]),
-                                 reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]),
-                                 reinterpret_cast<const ModalCoef<T>*>(pp.coef[2])};
+  const gptr<const cpx<T>> tw0 = (gptr<const cpx<T>>)pp.tw0;
+  const gptr<const ModalCoef<T>> coef[3] = {(gptr<const ModalCoef<T>>)pp.coef[0],
+                                            (gptr<const ModalCoef<T>>)pp.coef[1],
+                                            (gptr<const ModalCoef<T>>)pp.coef[2]};
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int W = pp.w, P = pp.p, N0 = pp.n0, PW = P * W;
   const int f0 = t * W;

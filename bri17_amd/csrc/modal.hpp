@@ -71,8 +71,8 @@ __device__ __forceinline__ void sym2_apply(T K00, T K11, T K01, bool null, cpx<T
 // Block at k = (k0, k1, kl) from the per-axis tables phi, chi, psi:
 //   H_ii = phi_i prod_{m != i} chi_m,   K_ii = s H_ii + mu sum_m H_mm,
 //   K_ij = s psi_i psi_j prod_{m not in {i,j}} chi_m   (2D: no chi product).
-template <typename T, int DIM, int MODE>
-__device__ __forceinline__ void modal_block(const ModalCoef<T>* const* coef, int k0, int k1,
+template <typename T, int DIM, int MODE, typename CP>
+__device__ __forceinline__ void modal_block(const CP* coef, int k0, int k1,
                                             int kl, T mu, T s, cpx<T>* x) {
   if constexpr (DIM == 2) {
     const ModalCoef<T> a = coef[0][k0], b = coef[1][kl];

@@ -18,6 +18,10 @@
 
 namespace b17 {
 
+// LDS per plane workgroup aimed for when choosing the middle-axis split: four
+// such workgroups (plus a little static LDS each) fit the 160 KiB of a CU.
+constexpr size_t kPlaneLds = 39 * 1024;
+
 static thread_local std::string t_err;
 void set_error(const std::string& msg) { t_err = msg; }
 const std::string& get_error() { return t_err; }
@@ -206,15 +210,20 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   if (dim == 3) {
     const int n1 = shape[1];
     // smallest split P of the middle axis whose Q = N1/P rows fit in LDS
+    // (preferably within kPlaneLds, so that several plane workgroups share a
+    // CU and the batch kernel's items stay small)
     int P = 0;
     const int cands[] = {1, 2, 3, 4, 5, 7, 8, 16};
-    for (int c : cands) {
-      if (tun.plane_split && c != tun.plane_split) continue;
-      if (n1 % c) continue;
-      if ((size_t)(n1 / c) * rsl * csz <= budget) {
-        P = c;
-        break;
+    for (size_t b : {std::min(budget, kPlaneLds), budget}) {
+      for (int c : cands) {
+        if (tun.plane_split && c != tun.plane_split) continue;
+        if (n1 % c) continue;
+        if ((size_t)(n1 / c) * rsl * csz <= b) {
+          P = c;
+          break;
+        }
       }
+      if (P) break;
     }
     if (!P) return fail(2, "no middle-axis split fits the LDS budget for this grid");
     const int Q = n1 / P;
@@ -286,6 +295,42 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   for (int d = 0; d < 3; d++) pc.coef[d] = d < dim ? pl->coef[d] : nullptr;
   out = pl;
   return Status{};
+}
+
+void batch_minimum(const Plan& pl, size_t& lds, int& threads) {
+  const size_t csz = 2 * pl.esz;
+  const PlaneParams& pp = pl.plane;
+  const PencilParams& pc = pl.pencil;
+  const size_t plane = pp.has_col ? (size_t)pp.q * pp.rsl * csz : (size_t)pp.rsl * csz;
+  const size_t pencil = (size_t)pl.dim * pc.n0 * pc.p * csz;
+  lds = std::max(plane, pencil);
+  int t = std::max({pp.row.tpt, pp.has_col ? pp.col.tpt : 1, pc.ax0.tpt});
+  threads = round_up(t, 64);
+}
+
+bool batch_geometry(const Plan& pl, size_t lds, PlaneParams& pp, int& plane_blocks,
+                    PencilParams& pc) {
+  const size_t csz = 2 * pl.esz;
+  pp = pl.plane;
+  plane_blocks = pl.plane_blocks;
+  if (!pp.has_col) {
+    int G = std::min<int>(pp.q, (int)(lds / ((size_t)pp.rsl * csz)));
+    if (G < 1) return false;
+    pp.q = G;
+    pp.n1 = G;
+    plane_blocks = (pp.nrows + G - 1) / G;
+  } else if ((size_t)pp.q * pp.rsl * csz > lds) {
+    return false;
+  }
+  const size_t per_w = (size_t)pl.dim * pc.n0 * pc.p * csz;
+  int W = pc.w;
+  while (W > 1 && per_w * W > lds) W /= 2;
+  if (per_w * W > lds) return false;
+  pc.w = W;
+  pc.ntiles = (pc.f + W - 1) / W;
+  pc.wdiv = make_fastdiv((uint32_t)W);
+  pc.pwdiv = make_fastdiv((uint32_t)(pc.p * W));
+  return true;
 }
 
 static std::string radices_json(const FftDesc& d) {

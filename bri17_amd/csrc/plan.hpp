@@ -67,6 +67,15 @@ struct Plan {
 Status build_plan(int dim, const int* shape, const double* L, int dtype,
                   const Tuning& tun, std::shared_ptr<Plan>& out);
 
+// Work items of a plan inside the persistent batch kernel: the same
+// arithmetic as the plan (radix orders, middle-axis split), with the 2D
+// plane row group and the pencil tile width narrowed so that every item fits
+// `lds` bytes.  false: a single row or column does not fit.
+bool batch_geometry(const Plan& pl, size_t lds, PlaneParams& pp, int& plane_blocks,
+                    PencilParams& pc);
+// Least LDS (bytes) and threads with which every item of the plan can run.
+void batch_minimum(const Plan& pl, size_t& lds, int& threads);
+
 // JSON summary of a plan's geometry (for logs and benchmarks).
 std::string describe_plan(const Plan& pl);
 

@@ -206,6 +206,102 @@ int bri17_tuning_cache_clear(void);
 int bri17_operator_profile(bri17_operator* op, int enable);
 int bri17_operator_profile_read(bri17_operator* op, int* calls, double* ms3);
 
+/* --------------------------------------------------------------- serving
+ * No reference counterpart (the reference applies one operator per call,
+ * tests/test_bri17.cpp:56-107).  A request loop on one device: requests of
+ * mixed grids, dtypes, materials and modes are submitted from any thread; a
+ * worker thread admits them in arrival order against an HBM budget and
+ * launches them in waves, each wave on its own stream, while earlier waves
+ * still run.  A wave is either one persistent batched launch of requests
+ * with runtime-size plans (bri17_batch_*), or one request that runs alone on
+ * its plan's own kernels (grids with fixed-size kernels, and grids of at
+ * least `solo_nodes` nodes).  Which of the two a request takes depends on the
+ * request alone, so its result does not depend on what it was served with.
+ * Plans (twiddle and modal tables) stay resident per (grid, dtype).
+ *
+ * HBM footprint of a request = its input + output fields + its half-spectrum
+ * workspace.  Admission holds the sum over admitted requests, plus the idle
+ * workspaces and staging buffers the server caches, within the budget; a
+ * request whose footprint alone exceeds the budget is rejected at submit.  */
+
+typedef struct bri17_server bri17_server;
+
+typedef struct bri17_server_config {
+  size_t hbm_budget_bytes; /* 0: half of the free device memory at creation */
+  int max_batch;           /* requests per batched wave (0: 64) */
+  int max_waves;           /* waves in flight, one stream each (0: 4, max 16) */
+  int64_t solo_nodes;      /* grids of at least this many nodes run alone
+                              (0: 2^21, -1: never) */
+  int spin_limit;          /* dependency-wait bound of batched waves (0: default) */
+} bri17_server_config;
+
+#define BRI17_MEM_DEVICE 0 /* in / out are device pointers owned by the caller */
+#define BRI17_MEM_HOST 1   /* in / out are host pointers; the server stages them */
+
+typedef struct bri17_request {
+  int dim;          /* 2 or 3 */
+  int shape[3];     /* N0, N1[, N2] */
+  double L[3];      /* cell sizes of the periodic box (L0, L1[, L2]) */
+  double mu, nu;    /* shear modulus and Poisson ratio of this request */
+  int mode;         /* 0: K u, 1: Green solve (as bri17_operator_apply / solve) */
+  int dtype;        /* BRI17_F32 or BRI17_F64 */
+  int memory;       /* BRI17_MEM_DEVICE or BRI17_MEM_HOST */
+  const void* in;   /* field (DIM, N0, ...); must stay valid until completion */
+  void* out;
+} bri17_request;
+
+#define BRI17_REQ_QUEUED 0
+#define BRI17_REQ_RUNNING 1
+#define BRI17_REQ_DONE 2
+#define BRI17_REQ_FAILED 3
+#define BRI17_REQ_CANCELLED 4
+
+typedef struct bri17_request_result {
+  int state;          /* BRI17_REQ_* */
+  int error;          /* status code of a failed request (0 otherwise) */
+  double queued_s;    /* submit -> launch */
+  double latency_s;   /* submit -> completion observed by the server */
+  int64_t wave;       /* wave the request ran in */
+  int wave_size;      /* requests in that wave */
+} bri17_request_result;
+
+typedef struct bri17_server_stats {
+  int64_t submitted, completed, failed, rejected, cancelled;
+  int64_t waves, batched_waves, solo_waves;
+  int64_t queued, running;
+  size_t hbm_budget, hbm_admitted, hbm_cached, hbm_peak;
+  int64_t plans_resident;
+  int64_t peak_waves_in_flight; /* most waves seen on the device at once */
+} bri17_server_stats;
+
+int bri17_server_create(const bri17_server_config* cfg, bri17_server** server);
+/* Waits for every admitted request, then frees everything. */
+int bri17_server_destroy(bri17_server* server);
+/* Queue a request; returns its id.  Fails at once (nothing queued) on an
+ * invalid request, an unsupported grid, or a footprint above the budget. */
+int bri17_server_submit(bri17_server* server, const bri17_request* req, int64_t* id);
+/* Queue n requests at once: they reach the worker together, so they can
+ * share waves.  All or nothing: if one is rejected, none is queued. */
+int bri17_server_submit_many(bri17_server* server, const bri17_request* reqs, int n,
+                             int64_t* ids);
+/* Non-blocking state of a request.  A request in a final state (done,
+ * failed, cancelled) is forgotten once its result has been returned. */
+int bri17_server_poll(bri17_server* server, int64_t id, bri17_request_result* res);
+/* Block until the request reaches a final state or `timeout_s` elapses
+ * (negative: no limit); BRI17_ERR_STATE on timeout. */
+int bri17_server_wait(bri17_server* server, int64_t id, double timeout_s,
+                      bri17_request_result* res);
+/* Cancel a request that is still queued (BRI17_ERR_STATE once launched). */
+int bri17_server_cancel(bri17_server* server, int64_t id);
+/* Block until nothing is queued or running. */
+int bri17_server_drain(bri17_server* server);
+int bri17_server_get_stats(bri17_server* server, bri17_server_stats* stats);
+/* Release the idle cached workspaces and staging buffers. */
+int bri17_server_trim(bri17_server* server);
+/* Dependency-wait bound of later batched waves (a wave whose wait runs out
+ * fails with BRI17_ERR_STATE; the server goes on with the next waves). */
+int bri17_server_set_spin_limit(bri17_server* server, int limit);
+
 #ifdef __cplusplus
 }
 #endif

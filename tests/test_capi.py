@@ -112,3 +112,48 @@ def test_pybri17_layout_errors_are_value_errors():
     with pytest.raises(ValueError):
         h.modal_eigenstress_to_opposite_strain(k, np.zeros((3, 1), complex), np.zeros(3, complex))
     assert repr(h) == "Hooke<d,2>{mu=1,nu=0.3,grid=CartesianGrid<d,2>{shape={4,4,},L={1,1,}}\n"
+
+
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    """sizeof / offsetof of every struct of the C ABI, compiled with the host
+    C compiler, equal those of the ctypes mirrors."""
+    import shutil
+    import subprocess
+    from bri17_amd import _lib
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    mirrors = {"bri17_batch_item": _lib.BatchItem, "bri17_tuning": _lib.Tuning,
+               "bri17_server_config": _lib.ServerConfig, "bri17_request": _lib.Request,
+               "bri17_request_result": _lib.RequestResult, "bri17_server_stats": _lib.ServerStats}
+    renames = {"inp": "in"}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bri17_amd.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, renames.get(f, f)))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                         text=True).stdout.split("\n") if l)
+    for cname, cls in mirrors.items():
+        assert int(out[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(out["%s.%s" % (cname, f)]) == getattr(cls, f).offset, (cname, f)
+
+
+def test_server_entry_points_validate_arguments():
+    from bri17_amd import _lib
+    L = _lib.lib()
+    rid = ctypes.c_int64()
+    assert L.bri17_server_submit(None, ctypes.byref(_lib.Request()), ctypes.byref(rid)) == _lib.BRI17_ERR_INVALID
+    assert L.bri17_server_poll(None, 1, None) == _lib.BRI17_ERR_INVALID
+    assert L.bri17_server_get_stats(None, None) == _lib.BRI17_ERR_INVALID
+    assert L.bri17_server_destroy(None) == _lib.BRI17_OK
+    import torch
+    if not torch.cuda.is_available():
+        h = ctypes.c_void_p()
+        assert L.bri17_server_create(None, ctypes.byref(h)) == _lib.BRI17_ERR_HIP
