@@ -240,6 +240,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched launch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run 3 plain device copies of MIB MiB each (reference kernel for "
+                         "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -250,6 +253,13 @@ def main():
     from bri17_amd import StiffnessMatrix
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration << 18, device=dev)  # MiB of float32
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     n = args.size
     shape = (n, n, n)
     mu, nu = 1.0, 0.25
@@ -333,6 +343,16 @@ def main():
         "green_solves_per_s": K / t_solve,
     }
     line.update(result)
+    # host-pointer entry point (copy in over PCIe, apply, copy out): not the
+    # headline, which starts from inputs resident in HBM
+    fh = f.cpu().numpy()
+    oh = fh.copy()
+    op.apply(fh, out=oh)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        op.apply(fh, out=oh)
+    line["host_pointer_applies_per_s"] = 3 / (time.perf_counter() - t0)
+    del fh, oh
     if not args.no_batch:
         line["batched_64"] = batched(dev, max(5, K // 10), 3)
     if not args.no_serve:
