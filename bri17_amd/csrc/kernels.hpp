@@ -66,9 +66,17 @@ struct PencilParams {
   double scale;   // |h|/|N| (apply) or 1/(|h||N|) (solve), for reference
   FastDiv wdiv, pwdiv, rsdiv;
   FftDesc ax0;
+  FftDesc pfft;         // P-point transform of a split p without a fused radix
+                        // kernel (twiddle stride q into tw1)
   const void* tw0;      // exp(-2 pi i j / n0)
+  const void* tw1;      // exp(-2 pi i j / n1) (3D)
   const void* coef[3];  // ModalCoef<T>[N_d] per axis
 };
+
+// Splits whose P-point DFT is one fused radix butterfly per column.
+__host__ __device__ constexpr bool fused_split(int p) {
+  return p == 1 || p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16;
+}
 
 template <typename T>
 hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp,

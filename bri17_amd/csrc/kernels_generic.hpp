@@ -283,7 +283,8 @@ __device__ __forceinline__ void pdft_r(cpx<T>* lds, int W, FastDiv wdiv, int ite
   }
 }
 
-// P-point DFT over the split index p for every (c, i0, w) of the tile.
+// P-point DFT over the split index p for every (c, i0, w) of the tile (the
+// caller synchronises before the results are read).
 template <typename T, bool INV>
 __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
                                      int nouter, int tid, int nthr) {
@@ -298,7 +299,26 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
     case 16: pdft_r<T, 16, INV>(lds, W, wdiv, items, tid, nthr); break;
     default: break;
   }
-  __syncthreads();
+}
+
+// P-point DFT for splits without a fused butterfly (P = 6, 9, 12, 27, ...):
+// a Stockham transform per (c, i0, w) column of the tile, P elements at
+// stride W, twiddles exp(-2 pi i j / P) = tw1[j q].
+template <typename T, bool INV>
+__device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const PencilParams& pp, int nouter,
+                                              int tid, int nthr) {
+  const gptr<const cpx<T>> tw1 = (gptr<const cpx<T>>)pp.tw1;
+  const int W = pp.w, P = pp.p, tpt = pp.pfft.tpt, S = nthr / tpt;
+  const int sl = tid % S, lane = tid / S;
+  const bool inr = tid < S * tpt;
+  const int nseq = nouter * W;
+  for (int s0 = 0; s0 < nseq; s0 += S) {
+    const int s = s0 + sl;
+    const bool act = inr && s < nseq;
+    const int outer = act ? (int)fdiv((uint32_t)s, pp.wdiv) : 0;
+    const int base = act ? outer * P * W + (s - outer * W) : 0;
+    lds_fft<T, INV, kMaxE>(lds, tw1, pp.pfft, base, W, lane, act);
+  }
 }
 
 template <typename T, int DIM, int MODE>
@@ -324,7 +344,13 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     return v;
   }, [&](int e, cpx<T> v) { lds[e] = v; });
   __syncthreads();
-  if (P > 1) pdft<T, false>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+  if (P > 1) {
+    if (fused_split(P))
+      pdft<T, false>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+    else
+      pdft_stockham<T, false>(lds, pp, DIM * N0, tid, nthr);
+    __syncthreads();  // the axis-0 passes read columns other threads wrote
+  }
 
   const int tpt = pp.ax0.tpt, S = nthr / tpt;
   const int sl = tid % S, lane = tid / S;
@@ -369,7 +395,13 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     const int base = act ? g * cst + (s - g * PW) : 0;
     lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
   }
-  if (P > 1) pdft<T, true>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+  if (P > 1) {
+    if (fused_split(P))
+      pdft<T, true>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+    else
+      pdft_stockham<T, true>(lds, pp, DIM * N0, tid, nthr);
+    __syncthreads();  // the store reads entries other threads wrote
+  }
 
   staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) { return lds[e]; }, [&](int e, cpx<T> v) {
     const int rest = (int)fdiv((uint32_t)e, pp.wdiv);

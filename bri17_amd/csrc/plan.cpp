@@ -209,16 +209,21 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
 
   if (dim == 3) {
     const int n1 = shape[1];
-    // smallest split P of the middle axis whose Q = N1/P rows fit in LDS
-    // (preferably within kPlaneLds, so that several plane workgroups share a
-    // CU and the batch kernel's items stay small)
+    // Smallest split P of the middle axis whose Q = N1/P rows fit in LDS,
+    // preferably within kPlaneLds (several plane workgroups share a CU and
+    // the batch kernel's items stay small), leaving room for a pencil tile
+    // of width 1 (DIM N0 P complex entries).  Any divisor up to 64 is a
+    // candidate; splits without a fused butterfly run a Stockham P-point
+    // transform in the pencil.
     int P = 0;
-    const int cands[] = {1, 2, 3, 4, 5, 7, 8, 16};
+    std::vector<int> cands = {1};
+    for (int c = 2; c <= 64 && c <= n1; c++)
+      if (n1 % c == 0 && !default_radices(c).empty()) cands.push_back(c);
     for (size_t b : {std::min(budget, kPlaneLds), budget}) {
       for (int c : cands) {
         if (tun.plane_split && c != tun.plane_split) continue;
         if (n1 % c) continue;
-        if ((size_t)(n1 / c) * rsl * csz <= b) {
+        if ((size_t)(n1 / c) * rsl * csz <= b && (size_t)dim * shape[0] * c * csz <= budget) {
           P = c;
           break;
         }
@@ -242,6 +247,8 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     pc.p = P;
     pc.q = Q;
     pc.f = Q * nlh;
+    if (!fused_split(P) && !make_fft_desc(P, Q, default_radices(P), pc.pfft, &why))
+      return fail(2, "split transform: " + why);
   } else {
     const int nrows = dim * shape[0];
     int G = std::max(1, std::min(256 / pp.row.tpt, (nrows + 255) / 256));
@@ -281,6 +288,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     int nseq = dim * pc.p * W;
     int nthr = std::min(1024, round_up(nseq * pc.ax0.tpt, 64));
     nthr = std::max(nthr, round_up(pc.ax0.tpt, 64));
+    if (!fused_split(pc.p)) nthr = std::max(nthr, round_up(pc.pfft.tpt, 64));
     pl->pencil_threads = nthr;
     pl->pencil_lds = per_w * W;
   }
@@ -292,6 +300,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   pp.tw_row = pl->tw[dim - 1];
   pp.tw_col = dim == 3 ? pl->tw[1] : nullptr;
   pc.tw0 = pl->tw[0];
+  pc.tw1 = dim == 3 ? pl->tw[1] : nullptr;
   for (int d = 0; d < 3; d++) pc.coef[d] = d < dim ? pl->coef[d] : nullptr;
   out = pl;
   return Status{};
@@ -304,7 +313,8 @@ void batch_minimum(const Plan& pl, size_t& lds, int& threads) {
   const size_t plane = pp.has_col ? (size_t)pp.q * pp.rsl * csz : (size_t)pp.rsl * csz;
   const size_t pencil = (size_t)pl.dim * pc.n0 * pc.p * csz;
   lds = std::max(plane, pencil);
-  int t = std::max({pp.row.tpt, pp.has_col ? pp.col.tpt : 1, pc.ax0.tpt});
+  int t = std::max({pp.row.tpt, pp.has_col ? pp.col.tpt : 1, pc.ax0.tpt,
+                    fused_split(pc.p) ? 1 : pc.pfft.tpt});
   threads = round_up(t, 64);
 }
 

@@ -93,7 +93,8 @@ int bri17_operator_destroy(bri17_operator* op);
  * mean "automatic". */
 typedef struct bri17_tuning {
   int plane_split;    /* 3D: split P of the middle axis between the plane and
-                         pencil kernels (1,2,3,4,5,7,8,16) */
+                         pencil kernels (a divisor of N1, at most 64, with
+                         prime factors <= 7) */
   int pencil_width;   /* contiguous spectrum entries per pencil tile */
   int lds_budget_kb;  /* LDS per workgroup, KiB (<= 160) */
   int xcd_remap;      /* 0: off, 1: on, -1: automatic (on) */

@@ -177,6 +177,37 @@ def test_config3_3d_256_fp32(device, oracle):
     assert e1 <= 1e-5 and e2 <= 1e-5
 
 
+@pytest.mark.parametrize("split", [6, 9, 18, 27])
+@pytest.mark.parametrize("mode", ["apply", "solve"])
+def test_middle_axis_splits_without_fused_butterfly(device, oracle, split, mode):
+    """Splits P whose P-point DFT runs as a Stockham transform in the pencil
+    kernel agree with the fp64 oracle."""
+    import torch
+    from bri17_amd import StiffnessMatrix
+    shape, L = (10, 54, 12), (1.0, 1.3, 0.7)
+    op = StiffnessMatrix(shape, L, 1.4, 0.27, "float64", tuning=dict(plane_split=split, force_generic=1))
+    assert op.describe()["plane"]["split"] == split
+    g = torch.Generator(device=device).manual_seed(split)
+    u = torch.rand((3,) + shape, generator=g, device=device, dtype=torch.float64) * 2 - 1
+    got = (op.apply(u) if mode == "apply" else op.solve(u)).cpu().numpy()
+    fn = oracle.apply if mode == "apply" else oracle.solve
+    ref = fn(shape, L, 1.4, 0.27, u.cpu().numpy())
+    assert rel_err(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_plans_exist_for_large_and_odd_grids(device, dtype):
+    """Every grid of the benchmark and serving mixes gets a plan whose
+    workgroups fit the 160 KiB of LDS of a CU (plan creation only)."""
+    from bri17_amd import StiffnessMatrix
+    shapes = [(256, 256, 256), (243, 243, 243), (256, 128, 64), (216, 256, 162), (512, 512),
+              (486, 324), (343, 49, 125), (7, 5, 3)]
+    for s in shapes:
+        d = StiffnessMatrix(s, None, 1.0, 0.3, dtype).describe()
+        print(s, dtype, d["plane"]["split"], d["plane"]["lds"], d["pencil"]["width"], d["pencil"]["lds"])
+        assert d["plane"]["lds"] <= 160 * 1024 and d["pencil"]["lds"] <= 160 * 1024
+
+
 def test_deterministic_and_linear(device, oracle):
     import torch
     shape = (48, 36, 54)
