@@ -251,11 +251,17 @@ int eigenstress(int dim, const int* shape, const T* L, double mu, double nu, con
 }
 
 
+// workspace bytes of a mode: the half spectrum of max(in, out) components
+size_t mode_ws_bytes(const b17::Plan& pl, int mode) {
+  const int c = std::max(b17::mode_cin(pl.dim, mode), b17::mode_cout(pl.dim, mode));
+  return pl.ws_bytes / pl.dim * c;
+}
+
 int run(bri17_operator* op, int mode, const void* in, void* out, void* stream) {
   if (!op || !in || !out) return err(BRI17_ERR_INVALID, "null argument");
   if (in == out) return err(BRI17_ERR_INVALID, "input and output must not alias");
   std::lock_guard<std::mutex> g(op->mtx);
-  int rc = b17_ensure_ws(op);
+  int rc = b17_ensure_ws(op, mode_ws_bytes(*op->plan, mode));
   if (rc) return rc;
   hipEvent_t* ev = nullptr;
   if (op->profiling) {
@@ -275,32 +281,54 @@ int run(bri17_operator* op, int mode, const void* in, void* out, void* stream) {
 
 int run_host(bri17_operator* op, int mode, const void* in, void* out) {
   if (!op || !in || !out) return err(BRI17_ERR_INVALID, "null argument");
-  const size_t bytes = (size_t)op->plan->dim * op->plan->nnodes * op->plan->esz;
-  {
-    std::lock_guard<std::mutex> g(op->mtx);
-    if (!op->host_in) {
-      if (hipMalloc(&op->host_in, bytes) != hipSuccess) return err(BRI17_ERR_NOMEM, "hipMalloc failed");
-      if (hipMalloc(&op->host_out, bytes) != hipSuccess) return err(BRI17_ERR_NOMEM, "hipMalloc failed");
-    }
-    if (hipMemcpy(op->host_in, in, bytes, hipMemcpyHostToDevice) != hipSuccess)
-      return err(BRI17_ERR_HIP, "hipMemcpy to device failed");
+  const b17::Plan& pl = *op->plan;
+  const size_t field = (size_t)pl.nnodes * pl.esz;
+  const size_t in_bytes = b17::mode_cin(pl.dim, mode) * field;
+  const size_t out_bytes = b17::mode_cout(pl.dim, mode) * field;
+  const size_t need = std::max(in_bytes, out_bytes);
+  std::lock_guard<std::mutex> hg(op->host_mtx);
+  if (op->host_bytes < need) {
+    // synchronous entry points: nothing is in flight on the old buffers
+    if (op->host_in) (void)hipFree(op->host_in);
+    if (op->host_out) (void)hipFree(op->host_out);
+    op->host_in = op->host_out = nullptr;
+    op->host_bytes = 0;
+    if (hipMalloc(&op->host_in, need) != hipSuccess || hipMalloc(&op->host_out, need) != hipSuccess)
+      return err(BRI17_ERR_NOMEM, "hipMalloc(host staging) failed");
+    op->host_bytes = need;
   }
+  if (hipMemcpy(op->host_in, in, in_bytes, hipMemcpyHostToDevice) != hipSuccess)
+    return err(BRI17_ERR_HIP, "hipMemcpy to device failed");
   int rc = run(op, mode, op->host_in, op->host_out, nullptr);
   if (rc) return rc;
-  hipError_t e = hipMemcpy(out, op->host_out, bytes, hipMemcpyDeviceToHost);
+  hipError_t e = hipMemcpy(out, op->host_out, out_bytes, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return err(BRI17_ERR_HIP, hipGetErrorString(e));
   return BRI17_OK;
 }
 
 }  // namespace
 
-int b17_ensure_ws(bri17_operator* op) {
-  if (op->ws) return BRI17_OK;
-  hipError_t e = hipMalloc(&op->ws, op->plan->ws_bytes);
+int b17_ensure_ws(bri17_operator* op, size_t bytes) {
+  if (bytes == 0) bytes = op->plan->ws_bytes;
+  if (op->ws && op->ws_bytes >= bytes) return BRI17_OK;
+  if (op->ws && op->ws_pinned)
+    return err(BRI17_ERR_STATE,
+               "the workspace is captured in a graph and cannot grow: run this mode once "
+               "before capturing");
+  if (op->ws) {
+    // growing (first strain / eigenstress call): earlier launches on any
+    // stream may still use the old workspace
+    if (hipDeviceSynchronize() != hipSuccess) return err(BRI17_ERR_HIP, "hipDeviceSynchronize failed");
+    (void)hipFree(op->ws);
+    op->ws = nullptr;
+    op->ws_bytes = 0;
+  }
+  hipError_t e = hipMalloc(&op->ws, bytes);
   if (e != hipSuccess) {
     op->ws = nullptr;
     return err(BRI17_ERR_NOMEM, "hipMalloc(workspace) failed");
   }
+  op->ws_bytes = bytes;
   return BRI17_OK;
 }
 
@@ -452,6 +480,18 @@ int bri17_operator_apply(bri17_operator* op, const void* u, void* Ku, void* stre
 }
 int bri17_operator_solve(bri17_operator* op, const void* f, void* u, void* stream) {
   return run(op, 1, f, u, stream);
+}
+int bri17_operator_strain(bri17_operator* op, const void* u, void* eps, void* stream) {
+  return run(op, 2, u, eps, stream);
+}
+int bri17_operator_eigenstress(bri17_operator* op, const void* tau, void* eta, void* stream) {
+  return run(op, 3, tau, eta, stream);
+}
+int bri17_operator_strain_host(bri17_operator* op, const void* u, void* eps) {
+  return run_host(op, 2, u, eps);
+}
+int bri17_operator_eigenstress_host(bri17_operator* op, const void* tau, void* eta) {
+  return run_host(op, 3, tau, eta);
 }
 int bri17_operator_apply_host(bri17_operator* op, const void* u, void* Ku) {
   return run_host(op, 0, u, Ku);

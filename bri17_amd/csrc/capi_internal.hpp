@@ -15,9 +15,13 @@ struct bri17_operator {
   std::shared_ptr<b17::Plan> plan;
   double mu = 1.0, nu = 0.0;
   void* ws = nullptr;         // half-spectrum workspace (device)
+  size_t ws_bytes = 0;
+  bool ws_pinned = false;     // captured in a graph: never reallocated
   void* host_in = nullptr;    // staging buffers of the *_host entry points
   void* host_out = nullptr;
+  size_t host_bytes = 0;
   std::mutex mtx;             // serialises use of the owned workspace
+  std::mutex host_mtx;        // serialises the *_host entry points
   // per-kernel timing (bri17_operator_profile)
   bool profiling = false;
   std::vector<hipEvent_t> events;  // 4 per profiled call
@@ -31,8 +35,9 @@ int b17_err(int code, const char* msg);
 // `tuning` NULL: the tuning registered for the grid, else the defaults
 int b17_acquire_plan(int dim, const int* shape, const double* L, int dtype,
                      const bri17_tuning* tuning, std::shared_ptr<b17::Plan>& out);
-// allocate the operator's workspace on first use
-int b17_ensure_ws(bri17_operator* op);
+// allocate (or grow) the operator's workspace; bytes 0: the plan's (DIM
+// components).  Call with op->mtx held.
+int b17_ensure_ws(bri17_operator* op, size_t bytes = 0);
 // launch the last batch of a context again (tables already on the device)
 int b17_batch_relaunch(bri17_batch* b, hipStream_t st);
 // workgroup shape of a persistent batched launch over these plans: 256

@@ -85,20 +85,28 @@ extern "C" {
 
 int bri17_graph_create(bri17_operator* op, int mode, const void* in, void* out,
                        bri17_graph** graph) {
-  if (!op || !in || !out || !graph || in == out || (mode != 0 && mode != 1))
+  if (!op || !in || !out || !graph || in == out || mode < 0 || mode > 3)
     return b17_err(BRI17_ERR_INVALID, "invalid argument");
   *graph = nullptr;
+  auto call = [&](hipStream_t st) {
+    switch (mode) {
+      case 0: return bri17_operator_apply(op, in, out, st);
+      case 1: return bri17_operator_solve(op, in, out, st);
+      case 2: return bri17_operator_strain(op, in, out, st);
+      default: return bri17_operator_eigenstress(op, in, out, st);
+    }
+  };
   // one eager run first: allocates the workspace and sets kernel attributes
   // outside the capture
-  int rc = mode == 0 ? bri17_operator_apply(op, in, out, nullptr)
-                     : bri17_operator_solve(op, in, out, nullptr);
+  int rc = call(nullptr);
   if (rc) return rc;
   if (hipDeviceSynchronize() != hipSuccess) return b17_err(BRI17_ERR_HIP, "device error");
-  return capture(
-      [&](hipStream_t st) {
-        return mode == 0 ? bri17_operator_apply(op, in, out, st) : bri17_operator_solve(op, in, out, st);
-      },
-      graph);
+  {
+    // the graph holds the workspace address from now on
+    std::lock_guard<std::mutex> g(op->mtx);
+    op->ws_pinned = true;
+  }
+  return capture(call, graph);
 }
 
 int bri17_batch_capture(bri17_batch* b, const bri17_batch_item* items, int n, bri17_graph** graph) {

@@ -89,11 +89,17 @@ static hipError_t launch_pencil_t(void* ws, const PencilParams& pp, int nthreads
 template <typename T>
 hipError_t launch_pencil(void* ws, const PencilParams& pp, int nthreads, size_t lds,
                          hipStream_t st) {
-  if (pp.dim == 2)
-    return pp.mode == 0 ? launch_pencil_t<T, 2, 0>(ws, pp, nthreads, lds, st)
-                        : launch_pencil_t<T, 2, 1>(ws, pp, nthreads, lds, st);
-  return pp.mode == 0 ? launch_pencil_t<T, 3, 0>(ws, pp, nthreads, lds, st)
-                      : launch_pencil_t<T, 3, 1>(ws, pp, nthreads, lds, st);
+  switch (pp.dim * 4 + pp.mode) {
+    case 8: return launch_pencil_t<T, 2, 0>(ws, pp, nthreads, lds, st);
+    case 9: return launch_pencil_t<T, 2, 1>(ws, pp, nthreads, lds, st);
+    case 10: return launch_pencil_t<T, 2, 2>(ws, pp, nthreads, lds, st);
+    case 11: return launch_pencil_t<T, 2, 3>(ws, pp, nthreads, lds, st);
+    case 12: return launch_pencil_t<T, 3, 0>(ws, pp, nthreads, lds, st);
+    case 13: return launch_pencil_t<T, 3, 1>(ws, pp, nthreads, lds, st);
+    case 14: return launch_pencil_t<T, 3, 2>(ws, pp, nthreads, lds, st);
+    case 15: return launch_pencil_t<T, 3, 3>(ws, pp, nthreads, lds, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 template hipError_t launch_plane_fwd<float>(const float*, void*, const PlaneParams&, int, int,

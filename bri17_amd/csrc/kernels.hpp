@@ -28,6 +28,33 @@ struct ModalCoef {
   T phi, chi, psi, pad;
 };
 
+// Per-axis strain-displacement factors (bri17.hpp:212-236): with
+// a = pi k / N and h = L / N, c = cos a, sn = sin a, sh = sin a / h.  The
+// vector B^(k) = 2i exp(i sum a) (sh_0 c_1 c_2, c_0 sh_1 c_2, c_0 c_1 sh_2)
+// (2D: (sh_0 c_1, c_0 sh_1)) is built from DIM entries, exp(i sum a) as the
+// product of the (c + i sn).
+template <typename T>
+struct StrainCoef {
+  T c, sn, sh, pad;
+};
+
+// Pencil modes: 0 K u, 1 Green solve (DIM -> DIM components); 2 strain of a
+// displacement field (DIM -> SYM Voigt-Mandel components); 3 opposite strain
+// induced by an eigenstress field (SYM -> SYM).
+template <int DIM, int MODE>
+struct ModeComps {
+  static constexpr int sym = DIM == 2 ? 3 : 6;
+  static constexpr int cin = MODE == 3 ? sym : DIM;
+  static constexpr int cout = MODE >= 2 ? sym : DIM;
+  static constexpr int cmax = cin > cout ? cin : cout;
+};
+__host__ __device__ constexpr int mode_cin(int dim, int mode) {
+  return mode == 3 ? (dim == 2 ? 3 : 6) : dim;
+}
+__host__ __device__ constexpr int mode_cout(int dim, int mode) {
+  return mode >= 2 ? (dim == 2 ? 3 : 6) : dim;
+}
+
 struct PlaneParams {
   int dim;
   int nl;        // row length (last axis)
@@ -40,7 +67,7 @@ struct PlaneParams {
   int n1;        // 3D: middle axis length; 2D: rows per workgroup
   int has_col;   // 3D: do the Q-point middle-axis stage
   int rs;        // workspace row stride (complex elements)
-  int nrows;     // 2D: total number of rows (DIM*N0); 3D unused
+  int nrows;     // 2D: total number of rows (components * N0); 3D unused
   FastDiv mdiv;  // division by m
   FastDiv hdiv;  // division by m/2+1 (pairs per row)
   FastDiv nlhdiv;
@@ -60,10 +87,11 @@ struct PencilParams {
   int f;        // flat extent of one p-block: q * rs
   int w;        // tile width (contiguous complex elements)
   int ntiles;
-  int mode;     // 0: K u, 1: K^-1 f
+  int mode;     // 0: K u, 1: K^-1 f, 2: strain B u, 3: eigenstress -> -strain
   int xcd_remap;
   double mu, s;   // material constants with the apply / solve scale folded in
   double scale;   // |h|/|N| (apply) or 1/(|h||N|) (solve), for reference
+  double bscale;  // modes 2 and 3: factor of the strain output (1/|N|, 2/|N|)
   FastDiv wdiv, pwdiv, rsdiv;
   FftDesc ax0;
   FftDesc pfft;         // P-point transform of a split p without a fused radix
@@ -71,6 +99,7 @@ struct PencilParams {
   const void* tw0;      // exp(-2 pi i j / n0)
   const void* tw1;      // exp(-2 pi i j / n1) (3D)
   const void* coef[3];  // ModalCoef<T>[N_d] per axis
+  const void* scoef[3]; // StrainCoef<T>[N_d] per axis
 };
 
 // Splits whose P-point DFT is one fused radix butterfly per column.

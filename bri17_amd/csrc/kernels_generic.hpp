@@ -324,6 +324,7 @@ __device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const PencilParams& p
 template <typename T, int DIM, int MODE>
 __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const PencilParams& pp, int t,
                                             cpx<T>* lds) {
+  constexpr int CIN = ModeComps<DIM, MODE>::cin, COUT = ModeComps<DIM, MODE>::cout;
   const gptr<const cpx<T>> tw0 = (gptr<const cpx<T>>)pp.tw0;
   const gptr<const ModalCoef<T>> coef[3] = {(gptr<const ModalCoef<T>>)pp.coef[0],
                                             (gptr<const ModalCoef<T>>)pp.coef[1],
@@ -332,11 +333,10 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
   const int W = pp.w, P = pp.p, N0 = pp.n0, PW = P * W;
   const int f0 = t * W;
   const int wv = pp.f - f0 < W ? pp.f - f0 : W;
-  const int tot = DIM * N0 * PW;
   const int cst = N0 * PW;  // LDS distance between components
 
   // load the tile: element (c, i0, p, w) at ((c N0 + i0) P + p) W + w
-  staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
+  staged_copy<kLoads<T>, cpx<T>>(CIN * N0 * PW, tid, nthr, [&](int e) {
     const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
     const int w = e - rest * W;
     cpx<T> v = {T(0), T(0)};
@@ -346,19 +346,18 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
   __syncthreads();
   if (P > 1) {
     if (fused_split(P))
-      pdft<T, false>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+      pdft<T, false>(lds, P, W, pp.wdiv, CIN * N0, tid, nthr);
     else
-      pdft_stockham<T, false>(lds, pp, DIM * N0, tid, nthr);
+      pdft_stockham<T, false>(lds, pp, CIN * N0, tid, nthr);
     __syncthreads();  // the axis-0 passes read columns other threads wrote
   }
 
   const int tpt = pp.ax0.tpt, S = nthr / tpt;
   const int sl = tid % S, lane = tid / S;
   const bool inr = tid < S * tpt;
-  const int nseq = DIM * PW;
-  for (int s0 = 0; s0 < nseq; s0 += S) {
+  for (int s0 = 0; s0 < CIN * PW; s0 += S) {
     const int s = s0 + sl;
-    const bool act = inr && s < nseq;
+    const bool act = inr && s < CIN * PW;
     const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
     const int base = act ? g * cst + (s - g * PW) : 0;
     lds_fft<T, false, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
@@ -378,32 +377,43 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
       const int kl = f - ka * pp.rs;
       if (kl >= pp.nlh) continue;
       const int k1 = ka + pp.q * kb;
-      cpx<T> x[DIM];
+      cpx<T> x[ModeComps<DIM, MODE>::cmax];
 #pragma unroll
-      for (int c = 0; c < DIM; c++) x[c] = lds[c * cst + e];
-      modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, x);
+      for (int c = 0; c < CIN; c++) x[c] = lds[c * cst + e];
+      if constexpr (MODE <= 1) {
+        modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, x);
+      } else {
+        const gptr<const StrainCoef<T>> scf[3] = {(gptr<const StrainCoef<T>>)pp.scoef[0],
+                                                  (gptr<const StrainCoef<T>>)pp.scoef[1],
+                                                  (gptr<const StrainCoef<T>>)pp.scoef[2]};
+        if constexpr (MODE == 2)
+          strain_block<T, DIM>(scf, k0, k1, kl, T(pp.bscale), x);
+        else
+          eigenstress_block<T, DIM>(coef, scf, k0, k1, kl, mu, sc, T(pp.bscale), x);
+      }
 #pragma unroll
-      for (int c = 0; c < DIM; c++) lds[c * cst + e] = x[c];
+      for (int c = 0; c < COUT; c++) lds[c * cst + e] = x[c];
     }
   }
   __syncthreads();
 
-  for (int s0 = 0; s0 < nseq; s0 += S) {
+  for (int s0 = 0; s0 < COUT * PW; s0 += S) {
     const int s = s0 + sl;
-    const bool act = inr && s < nseq;
+    const bool act = inr && s < COUT * PW;
     const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
     const int base = act ? g * cst + (s - g * PW) : 0;
     lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
   }
   if (P > 1) {
     if (fused_split(P))
-      pdft<T, true>(lds, P, W, pp.wdiv, DIM * N0, tid, nthr);
+      pdft<T, true>(lds, P, W, pp.wdiv, COUT * N0, tid, nthr);
     else
-      pdft_stockham<T, true>(lds, pp, DIM * N0, tid, nthr);
+      pdft_stockham<T, true>(lds, pp, COUT * N0, tid, nthr);
     __syncthreads();  // the store reads entries other threads wrote
   }
 
-  staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) { return lds[e]; }, [&](int e, cpx<T> v) {
+  staged_copy<kLoads<T>, cpx<T>>(COUT * N0 * PW, tid, nthr, [&](int e) { return lds[e]; },
+                                 [&](int e, cpx<T> v) {
     const int rest = (int)fdiv((uint32_t)e, pp.wdiv);
     const int w = e - rest * W;
     if (w < wv) ws[(int64_t)rest * pp.f + f0 + w] = v;

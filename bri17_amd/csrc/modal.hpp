@@ -91,4 +91,96 @@ __device__ __forceinline__ void modal_block(const CP* coef, int k0, int k1,
   }
 }
 
+// exp(i sum a) and the real factors beta of B^ = 2i exp(i sum a) beta at k.
+template <typename T, int DIM, typename SP>
+__device__ __forceinline__ cpx<T> strain_factors(const SP* sc, int k0, int k1, int kl, T* beta) {
+  if constexpr (DIM == 2) {
+    const StrainCoef<T> a = sc[0][k0], b = sc[1][kl];
+    beta[0] = a.sh * b.c;
+    beta[1] = a.c * b.sh;
+    return cmul(cpx<T>{a.c, a.sn}, cpx<T>{b.c, b.sn});
+  } else {
+    const StrainCoef<T> a = sc[0][k0], b = sc[1][k1], c = sc[2][kl];
+    beta[0] = a.sh * b.c * c.c;
+    beta[1] = a.c * b.sh * c.c;
+    beta[2] = a.c * b.c * c.sh;
+    return cmul(cmul(cpx<T>{a.c, a.sn}, cpx<T>{b.c, b.sn}), cpx<T>{c.c, c.sn});
+  }
+}
+
+// Voigt-Mandel order of the symmetric tensor components
+// ([xx, yy, sqrt2 xy] / [xx, yy, zz, sqrt2 yz, sqrt2 zx, sqrt2 xy],
+// tests/test_bri17.cpp:209-228, bri17.hpp:324-353)
+template <int DIM>
+struct Voigt {
+  static constexpr int n = DIM == 2 ? 3 : 6;
+  static constexpr int a[6] = {0, 1, DIM == 2 ? 0 : 2, 1, 2, 0};
+  static constexpr int b[6] = {0, 1, DIM == 2 ? 1 : 2, 2, 0, 1};
+  __host__ __device__ static constexpr bool shear(int i) { return DIM == 2 ? i == 2 : i >= 3; }
+};
+
+// Mode 2: eps^ = sym(B^ u^) (tests/test_bri17.cpp:196-228) times bscale, in
+// Voigt-Mandel order.  With B^ = 2i E beta: eps_ab = i E (beta_a u_b + beta_b u_a).
+template <typename T, int DIM, typename SP>
+__device__ __forceinline__ void strain_block(const SP* sc, int k0, int k1, int kl, T bscale,
+                                             cpx<T>* x) {
+  constexpr T r2 = T(1.4142135623730950488016887242096981L);
+  T beta[3];
+  const cpx<T> E = strain_factors<T, DIM>(sc, k0, k1, kl, beta);
+  const cpx<T> Z = {-E.y * bscale, E.x * bscale};  // i E bscale
+  cpx<T> u[DIM];
+#pragma unroll
+  for (int c = 0; c < DIM; c++) u[c] = x[c];
+#pragma unroll
+  for (int i = 0; i < Voigt<DIM>::n; i++) {
+    const int a = Voigt<DIM>::a[i], b = Voigt<DIM>::b[i];
+    const cpx<T> w = u[b] * beta[a] + u[a] * beta[b];
+    x[i] = cmul(Z, Voigt<DIM>::shear(i) ? w * r2 : w);
+  }
+}
+
+// Mode 3: bri17.hpp:308-355 at k, times bscale.  rhs = tau_mat conj(B^),
+// u = K^-1 rhs, eta = sym(B^ u); with B^ = 2i E beta and |E| = 1:
+// eta_ab = 2 (beta_a v_b + beta_b v_a), v = K^-1 (tau_mat beta).  Zero at k = 0.
+template <typename T, int DIM, typename CP, typename SP>
+__device__ __forceinline__ void eigenstress_block(const CP* coef, const SP* sc, int k0, int k1,
+                                                  int kl, T mu, T s, T bscale, cpx<T>* x) {
+  constexpr T r2 = T(1.4142135623730950488016887242096981L);
+  constexpr T ir2 = T(0.70710678118654752440084436210484903L);
+  constexpr int S = Voigt<DIM>::n;
+  const bool null = DIM == 2 ? (k0 == 0 && kl == 0) : (k0 == 0 && k1 == 0 && kl == 0);
+  T beta[3];
+  (void)strain_factors<T, DIM>(sc, k0, k1, kl, beta);
+  // v = tau_mat beta
+  cpx<T> t[3][3];
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    const int a = Voigt<DIM>::a[i], b = Voigt<DIM>::b[i];
+    const cpx<T> v = Voigt<DIM>::shear(i) ? x[i] * ir2 : x[i];
+    t[a][b] = v;
+    t[b][a] = v;
+  }
+  cpx<T> v[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; a++) {
+    cpx<T> acc = t[a][0] * beta[0];
+#pragma unroll
+    for (int b = 1; b < DIM; b++) acc = acc + t[a][b] * beta[b];
+    v[a] = acc;
+  }
+  modal_block<T, DIM, 1>(coef, k0, k1, kl, mu, s, v);
+  if (null) {
+#pragma unroll
+    for (int i = 0; i < S; i++) x[i] = cpx<T>{T(0), T(0)};
+    return;
+  }
+  const T f = T(2) * bscale;
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    const int a = Voigt<DIM>::a[i], b = Voigt<DIM>::b[i];
+    const cpx<T> w = v[b] * beta[a] + v[a] * beta[b];
+    x[i] = Voigt<DIM>::shear(i) ? w * (f * r2) : w * f;
+  }
+}
+
 }  // namespace b17

@@ -95,6 +95,7 @@ Plan::~Plan() {
   for (int d = 0; d < 3; d++) {
     if (tw[d]) (void)hipFree(tw[d]);
     if (coef[d]) (void)hipFree(coef[d]);
+    if (scoef[d]) (void)hipFree(scoef[d]);
   }
 }
 
@@ -121,7 +122,21 @@ static Status upload_tables(Plan& pl) {
       cf[4 * k + 2] = (T)(sinl(b) / h);
       cf[4 * k + 3] = (T)0;
     }
-    hipError_t e = hipMalloc(&pl.tw[d], tw.size() * sizeof(T));
+    // strain-displacement factors c = cos a, sn = sin a, sh = sin a / h with
+    // a = pi k / N (bri17.hpp:216-224)
+    std::vector<T> sf(4 * (size_t)n);
+    for (int k = 0; k < n; k++) {
+      const long double a = PI * (long double)k / (long double)n;
+      sf[4 * k + 0] = (T)cosl(a);
+      sf[4 * k + 1] = (T)sinl(a);
+      sf[4 * k + 2] = (T)(sinl(a) / h);
+      sf[4 * k + 3] = (T)0;
+    }
+    hipError_t e = hipMalloc(&pl.scoef[d], sf.size() * sizeof(T));
+    if (e != hipSuccess) return fail(3, std::string("hipMalloc(strain table): ") + hipGetErrorString(e));
+    e = hipMemcpy(pl.scoef[d], sf.data(), sf.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(3, std::string("hipMemcpy(strain table): ") + hipGetErrorString(e));
+    e = hipMalloc(&pl.tw[d], tw.size() * sizeof(T));
     if (e != hipSuccess) return fail(3, std::string("hipMalloc(twiddles): ") + hipGetErrorString(e));
     e = hipMalloc(&pl.coef[d], cf.size() * sizeof(T));
     if (e != hipSuccess) return fail(3, std::string("hipMalloc(modal table): ") + hipGetErrorString(e));
@@ -302,6 +317,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   pc.tw0 = pl->tw[0];
   pc.tw1 = dim == 3 ? pl->tw[1] : nullptr;
   for (int d = 0; d < 3; d++) pc.coef[d] = d < dim ? pl->coef[d] : nullptr;
+  for (int d = 0; d < 3; d++) pc.scoef[d] = d < dim ? pl->scoef[d] : nullptr;
   out = pl;
   return Status{};
 }
@@ -385,11 +401,98 @@ PencilParams pencil_for(const Plan& pl, double mu, double nu, int mode) {
   const double s = mu / (1.0 - 2.0 * nu);
   pc.mu = mode == 0 ? mu * pc.scale : mu / pc.scale;
   pc.s = mode == 0 ? s * pc.scale : s / pc.scale;
+  if (mode >= 2) {
+    // strain / eigenstress: the per-k map of the reference (K^ itself,
+    // bri17.hpp:247-292), inverse DFT divided by |N| (tests/test_bri17.cpp:230-237)
+    pc.scale = 1.0 / N;
+    pc.mu = mu;
+    pc.s = s;
+    pc.bscale = 1.0 / N;
+  }
   return pc;
+}
+
+Status mode_geometry(const Plan& pl, double mu, double nu, int mode, ModeGeometry& g) {
+  const int cin = mode_cin(pl.dim, mode), cout = mode_cout(pl.dim, mode);
+  const int cmax = std::max(cin, cout);
+  const size_t csz = 2 * pl.esz;
+  auto plane = [&](int ncomp, PlaneParams& pp, int& blocks) {
+    pp = pl.plane;
+    if (pp.has_col) {
+      blocks = ncomp * pl.shape[0] * pp.p;
+    } else {
+      pp.nrows = ncomp * pl.shape[0];
+      blocks = (pp.nrows + pp.q - 1) / pp.q;
+    }
+  };
+  plane(cin, g.fwd, g.fwd_blocks);
+  plane(cout, g.inv, g.inv_blocks);
+  g.plane_threads = pl.plane_threads;
+  g.plane_lds = pl.plane_lds;
+  g.pc = pencil_for(pl, mu, nu, mode);
+  g.pencil_threads = pl.pencil_threads;
+  g.pencil_lds = pl.pencil_lds;
+  g.ws_bytes = pl.ws_bytes;
+  if (mode <= 1) return Status{};
+  PencilParams& pc = g.pc;
+  const size_t per_w = (size_t)cmax * pc.n0 * pc.p * csz;
+  const size_t budget = std::min<size_t>(pl.tuning.lds_budget, 160 * 1024);
+  int W = pc.w;
+  while (W > 1 && per_w * W > budget) W /= 2;
+  if (per_w * W > budget)
+    return fail(2, "the pencil tile of this grid does not fit the LDS budget with " +
+                       std::to_string(cmax) + " components");
+  pc.w = W;
+  pc.ntiles = (pc.f + W - 1) / W;
+  pc.wdiv = make_fastdiv((uint32_t)W);
+  pc.pwdiv = make_fastdiv((uint32_t)(pc.p * W));
+  int nthr = std::min(1024, round_up(cmax * pc.p * W * pc.ax0.tpt, 64));
+  nthr = std::max(nthr, round_up(pc.ax0.tpt, 64));
+  if (!fused_split(pc.p)) nthr = std::max(nthr, round_up(pc.pfft.tpt, 64));
+  g.pencil_threads = nthr;
+  g.pencil_lds = per_w * W;
+  g.ws_bytes = pl.ws_bytes / pl.dim * cmax;
+  return Status{};
+}
+
+// Modes 2 and 3: runtime-size kernels over the mode's component counts.
+static Status run_mode(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
+                       void* ws, hipStream_t st, hipEvent_t* ev) {
+  ModeGeometry g;
+  Status s = mode_geometry(pl, mu, nu, mode, g);
+  if (!s.ok()) return s;
+  hipError_t e = hipSuccess;
+  auto mark = [&](int i) {
+    if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
+  };
+  mark(0);
+  if (pl.dtype == 0) {
+    e = launch_plane_fwd<float>((const float*)in, ws, g.fwd, g.fwd_blocks, g.plane_threads,
+                                g.plane_lds, st);
+    mark(1);
+    if (e == hipSuccess) e = launch_pencil<float>(ws, g.pc, g.pencil_threads, g.pencil_lds, st);
+    mark(2);
+    if (e == hipSuccess)
+      e = launch_plane_inv<float>(ws, (float*)out, g.inv, g.inv_blocks, g.plane_threads,
+                                  g.plane_lds, st);
+  } else {
+    e = launch_plane_fwd<double>((const double*)in, ws, g.fwd, g.fwd_blocks, g.plane_threads,
+                                 g.plane_lds, st);
+    mark(1);
+    if (e == hipSuccess) e = launch_pencil<double>(ws, g.pc, g.pencil_threads, g.pencil_lds, st);
+    mark(2);
+    if (e == hipSuccess)
+      e = launch_plane_inv<double>(ws, (double*)out, g.inv, g.inv_blocks, g.plane_threads,
+                                   g.plane_lds, st);
+  }
+  mark(3);
+  if (e != hipSuccess) return fail(3, std::string("kernel launch: ") + hipGetErrorString(e));
+  return Status{};
 }
 
 Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* in, void* out,
                     void* ws, hipStream_t st, hipEvent_t* ev) {
+  if (mode >= 2) return run_mode(pl, mu, nu, mode, in, out, ws, st, ev);
   const PencilParams pc = pencil_for(pl, mu, nu, mode);
   hipError_t e = hipSuccess;
   if (pl.use_static) {

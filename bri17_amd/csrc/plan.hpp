@@ -50,6 +50,7 @@ struct Plan {
   // device tables (owned)
   void* tw[3] = {nullptr, nullptr, nullptr};
   void* coef[3] = {nullptr, nullptr, nullptr};
+  void* scoef[3] = {nullptr, nullptr, nullptr};  // strain-displacement factors
 
   PlaneParams plane{};
   int plane_blocks = 0, plane_threads = 0;
@@ -83,6 +84,22 @@ std::string describe_plan(const Plan& pl);
 // Pencil parameters of a plan for one call: mode (0 apply, 1 solve) and the
 // material constants with the mode's scale folded in.
 PencilParams pencil_for(const Plan& pl, double mu, double nu, int mode);
+
+// Launch geometry of one mode.  Modes 0 and 1 use the plan's own geometry;
+// modes 2 (strain, DIM -> SYM components) and 3 (eigenstress, SYM -> SYM)
+// run the runtime-size kernels over their component counts, with the pencil
+// tile narrowed to hold max(in, out) components.
+struct ModeGeometry {
+  PlaneParams fwd, inv;
+  int fwd_blocks = 0, inv_blocks = 0;
+  int plane_threads = 0;
+  size_t plane_lds = 0;
+  PencilParams pc{};
+  int pencil_threads = 0;
+  size_t pencil_lds = 0;
+  size_t ws_bytes = 0;
+};
+Status mode_geometry(const Plan& pl, double mu, double nu, int mode, ModeGeometry& g);
 
 // `ev`, if not null, holds 4 events recorded before the first kernel and after
 // each of the three kernels (per-kernel timing).

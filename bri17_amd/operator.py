@@ -3,7 +3,11 @@
 Python face of ``bri17_operator`` (include/bri17_amd.h).  ``apply`` is the
 ``K u`` of StiffnessMatrixFactory::compute_Ku (tests/test_bri17.cpp:56-107);
 ``solve`` is the Green solve ``K u = f`` with the zero-mean convention of
-sphinx/theory.rst:208-212.  Fields have shape (DIM, N0, ..., N_{DIM-1}).
+sphinx/theory.rst:208-212; ``strain`` is ``B u``
+(tests/test_bri17.cpp:194-238); ``eigenstress`` maps an eigenstress field to
+the opposite of the strain it induces (bri17.hpp:308-355).  Displacement and
+force fields have shape (DIM, N0, ..., N_{DIM-1}); strain and stress fields
+(SYM, N0, ...), SYM = 3 (2D) or 6 (3D), Voigt-Mandel order.
 
 Device tensors (torch, on a ROCm device) run asynchronously on the current
 stream; numpy arrays go through the synchronous host-pointer entry points.
@@ -15,6 +19,9 @@ import numpy as np
 from . import _lib
 
 _DTYPES = {"float32": _lib.BRI17_F32, "float64": _lib.BRI17_F64}
+_MODES = {"apply": 0, "solve": 1, "strain": 2, "eigenstress": 3}
+_ENTRY = {0: "bri17_operator_apply", 1: "bri17_operator_solve", 2: "bri17_operator_strain",
+          3: "bri17_operator_eigenstress"}
 
 
 class StiffnessMatrix:
@@ -92,52 +99,62 @@ class StiffnessMatrix:
         return (self.dim,) + self.shape
 
     # --------------------------------------------------------------- compute
+    @property
+    def sym(self):
+        """Number of Voigt-Mandel components of a symmetric tensor field."""
+        return 3 if self.dim == 2 else 6
+
+    def _shapes(self, mode):
+        cin = self.sym if mode == 3 else self.dim
+        cout = self.sym if mode >= 2 else self.dim
+        return (cin,) + self.shape, (cout,) + self.shape
+
     def _run(self, x, out, mode, stream):
         L = _lib.lib()
+        ishape, oshape = self._shapes(mode)
+        name = _ENTRY[mode]
         if isinstance(x, np.ndarray):
             want = np.dtype(self.dtype)
-            if x.dtype != want or x.shape != self.field_shape:
-                raise ValueError("expected a %s array of shape %s" % (want.name, self.field_shape))
+            if x.dtype != want or x.shape != ishape:
+                raise ValueError("expected a %s array of shape %s" % (want.name, ishape))
             x = np.ascontiguousarray(x)
             if out is None:
-                out = np.empty_like(x)
-            elif out.dtype != want or out.shape != self.field_shape or not out.flags.c_contiguous:
-                raise ValueError("out must be a contiguous %s array of shape %s" % (want.name, self.field_shape))
-            fn = L.bri17_operator_apply_host if mode == 0 else L.bri17_operator_solve_host
-            _lib.check(fn(self._h, x.ctypes.data, out.ctypes.data))
+                out = np.empty(oshape, dtype=want)
+            elif out.dtype != want or out.shape != oshape or not out.flags.c_contiguous:
+                raise ValueError("out must be a contiguous %s array of shape %s" % (want.name, oshape))
+            _lib.check(getattr(L, name + "_host")(self._h, x.ctypes.data, out.ctypes.data))
             return out
         import torch  # device path
         if not isinstance(x, torch.Tensor) or not x.is_cuda:
             raise TypeError("expected a numpy array or a device torch.Tensor")
         want = getattr(torch, self.dtype)
-        if x.dtype != want or tuple(x.shape) != self.field_shape or not x.is_contiguous():
-            raise ValueError("expected a contiguous %s tensor of shape %s" % (self.dtype, self.field_shape))
+        if x.dtype != want or tuple(x.shape) != ishape or not x.is_contiguous():
+            raise ValueError("expected a contiguous %s tensor of shape %s" % (self.dtype, ishape))
         if out is None:
-            out = torch.empty_like(x)
-        elif (out.dtype != want or tuple(out.shape) != self.field_shape or not out.is_contiguous()
+            out = torch.empty(oshape, dtype=want, device=x.device)
+        elif (out.dtype != want or tuple(out.shape) != oshape or not out.is_contiguous()
               or out.device != x.device):
             raise ValueError("bad out tensor")
         if stream is None:
             stream = torch.cuda.current_stream(x.device).cuda_stream
-        fn = L.bri17_operator_apply if mode == 0 else L.bri17_operator_solve
         with torch.cuda.device(x.device):
-            _lib.check(fn(self._h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
-                          ctypes.c_void_p(stream)))
+            _lib.check(getattr(L, name)(self._h, ctypes.c_void_p(x.data_ptr()),
+                                        ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(stream)))
         return out
 
     def capture(self, x, out, mode="apply"):
-        """HIP graph of one apply / solve on fixed device tensors (runs once
-        eagerly, then captures); replay with ``Graph.launch``."""
+        """HIP graph of one call on fixed device tensors (runs once eagerly,
+        then captures); replay with ``Graph.launch``."""
         import torch
+        m = _MODES[mode] if isinstance(mode, str) else int(mode)
         want = getattr(torch, self.dtype)
-        for t in (x, out):
+        for t, shp in zip((x, out), self._shapes(m)):
             if (not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != want
-                    or tuple(t.shape) != self.field_shape or not t.is_contiguous()):
-                raise ValueError("expected contiguous %s device tensors of shape %s" % (self.dtype, self.field_shape))
+                    or tuple(t.shape) != shp or not t.is_contiguous()):
+                raise ValueError("expected contiguous %s device tensors of shape %s" % (self.dtype, shp))
         h = ctypes.c_void_p()
         with torch.cuda.device(x.device):
-            _lib.check(_lib.lib().bri17_graph_create(self._h, 0 if mode == "apply" else 1,
-                                                     ctypes.c_void_p(x.data_ptr()),
+            _lib.check(_lib.lib().bri17_graph_create(self._h, m, ctypes.c_void_p(x.data_ptr()),
                                                      ctypes.c_void_p(out.data_ptr()), ctypes.byref(h)))
         return Graph(h, keep=(self, x, out))
 
@@ -148,6 +165,16 @@ class StiffnessMatrix:
     def solve(self, f, out=None, stream=None):
         """u with K u = f - mean(f) and mean(u) = 0 (sphinx/theory.rst:208-212)."""
         return self._run(f, out, 1, stream)
+
+    def strain(self, u, out=None, stream=None):
+        """eps = B u, (SYM, N...) Voigt-Mandel cell field
+        (tests/test_bri17.cpp:194-238)."""
+        return self._run(u, out, 2, stream)
+
+    def eigenstress(self, tau, out=None, stream=None):
+        """Opposite of the strain induced by the eigenstress field tau,
+        (SYM, N...) -> (SYM, N...) (bri17.hpp:308-355 at every frequency)."""
+        return self._run(tau, out, 3, stream)
 
     __matmul__ = apply
 

@@ -214,9 +214,27 @@ class StiffnessMatrix {
     detail::check(bri17_operator_solve(op_, f, u, stream));
   }
 
+  /** Strain eps = B u of a nodal displacement field (tests/test_bri17.cpp:194-238):
+   *  u (DIM, N...) -> eps (SYM, N...) cell field, Voigt-Mandel order. */
+  void strain(T const *u, T *eps, void *stream = nullptr) const {
+    detail::check(bri17_operator_strain(op_, u, eps, stream));
+  }
+
+  /** Opposite of the strain induced by an eigenstress cell field:
+   *  Hooke::modal_eigenstress_to_opposite_strain at every frequency. */
+  void eigenstress_to_opposite_strain(T const *tau, T *eta, void *stream = nullptr) const {
+    detail::check(bri17_operator_eigenstress(op_, tau, eta, stream));
+  }
+
   /** Host-pointer variants (synchronous). */
   void apply_host(T const *u, T *Ku) const { detail::check(bri17_operator_apply_host(op_, u, Ku)); }
   void solve_host(T const *f, T *u) const { detail::check(bri17_operator_solve_host(op_, f, u)); }
+  void strain_host(T const *u, T *eps) const {
+    detail::check(bri17_operator_strain_host(op_, u, eps));
+  }
+  void eigenstress_to_opposite_strain_host(T const *tau, T *eta) const {
+    detail::check(bri17_operator_eigenstress_host(op_, tau, eta));
+  }
 
   bri17_operator *handle() const { return op_; }
 

@@ -141,6 +141,28 @@ int bri17_operator_solve(bri17_operator* op, const void* f, void* u, void* strea
 int bri17_operator_apply_host(bri17_operator* op, const void* u, void* Ku);
 int bri17_operator_solve_host(bri17_operator* op, const void* f, void* u);
 
+/* Strain of a displacement field, eps = B u: the operator of the reference's
+ * StrainDisplacementMatrixFactory::compute_Bu (tests/test_bri17.cpp:194-238)
+ * -- DFT of u, eps^(k) = sym(B^(k) u^(k)) with B^ from
+ * Hooke::modal_strain_displacement (bri17.hpp:212-236), inverse DFT / |N|.
+ * u: (DIM, N...) nodal field; eps: (SYM, N...) cell field, SYM = 3 (2D) or 6
+ * (3D), Voigt-Mandel order [xx, yy, sqrt2 xy] / [xx, yy, zz, sqrt2 yz,
+ * sqrt2 zx, sqrt2 xy].  Device pointers, asynchronous on `stream`. */
+int bri17_operator_strain(bri17_operator* op, const void* u, void* eps, void* stream);
+
+/* Opposite of the strain induced by an eigenstress field: DFT of tau,
+ * Hooke::modal_eigenstress_to_opposite_strain (bri17.hpp:308-355) at every
+ * frequency (the per-k loop of python/demo.py:24-40), inverse DFT / |N|.
+ * tau, eta: (SYM, N...) cell fields, Voigt-Mandel order.  Device pointers,
+ * asynchronous on `stream`.  The first strain / eigenstress call grows the
+ * operator's workspace to SYM components (not possible once the operator has
+ * been captured in a graph). */
+int bri17_operator_eigenstress(bri17_operator* op, const void* tau, void* eta, void* stream);
+
+/* Host-pointer variants (synchronous). */
+int bri17_operator_strain_host(bri17_operator* op, const void* u, void* eps);
+int bri17_operator_eigenstress_host(bri17_operator* op, const void* tau, void* eta);
+
 /* ---------------------------------------------------------------- batches */
 
 /* Many operators -- different grids, dimensions and materials, one dtype --
@@ -171,10 +193,11 @@ int bri17_batch_info(const bri17_batch* b, int* work_items, int* workgroups, int
 
 /* ----------------------------------------------------------------- graphs */
 
-/* A captured HIP graph of one apply / solve on fixed device buffers, or of
+/* A captured HIP graph of one operator call on fixed device buffers, or of
  * one batch, replayed with a single launch (no reference counterpart).  The
  * operator, batch context and buffers must outlive the graph; replays reuse
- * the operator's workspace, so replays of one graph must not overlap. */
+ * the operator's workspace, so replays of one graph must not overlap.
+ * mode: 0 apply, 1 solve, 2 strain, 3 eigenstress. */
 typedef struct bri17_graph bri17_graph;
 /* Runs the call once eagerly, then captures it. */
 int bri17_graph_create(bri17_operator* op, int mode, const void* in, void* out,

@@ -1,12 +1,13 @@
-// The reference's "Global assembly tests" (tests/test_bri17.cpp:318-536),
+// The reference's "Global assembly tests" (tests/test_bri17.cpp:318-605),
 // rewritten against the drop-in header include/bri17/bri17.hpp.
 //
 // Usage:
 //   test_dropin host <golden_dir>    grid and per-frequency checks (no GPU);
 //                                    prints per-k values for the Python side
-//   test_dropin device <golden_dir>  2D / 3D stiffness sections: the operator
-//                                    probed column by column on the GPU
-//                                    against the assembled Maxima Ke.
+//   test_dropin device <golden_dir>  2D / 3D stiffness and strain sections:
+//                                    the operators probed column by column on
+//                                    the GPU against the assembled Maxima Ke
+//                                    and Be.
 // Exit status 0 on success.
 #include <cmath>
 #include <complex>
@@ -25,20 +26,20 @@ static std::vector<double> read_values(const std::string &path) {
   return v;
 }
 
-// tests/test_bri17.cpp:9-28
+// tests/test_bri17.cpp:9-28 (rows x cols, row-major)
 static int assert_equal(const std::vector<double> &expected, const std::vector<double> &actual,
-                        int n, double rtol, double atol, const char *what) {
+                        int rows, int cols, double rtol, double atol, const char *what) {
   int bad = 0;
-  for (int i = 0; i < n * n; i++) {
+  for (int i = 0; i < rows * cols; i++) {
     double tol = rtol * std::fabs(expected[i]) + atol;
     if (!(std::fabs(actual[i] - expected[i]) <= tol)) {
       if (bad < 5)
-        std::printf("%s [%d, %d]: expected = %.17g, actual = %.17g\n", what, i / n, i % n,
+        std::printf("%s [%d, %d]: expected = %.17g, actual = %.17g\n", what, i / cols, i % cols,
                     expected[i], actual[i]);
       bad++;
     }
   }
-  std::printf("%s: %d of %d entries out of tolerance\n", what, bad, n * n);
+  std::printf("%s: %d of %d entries out of tolerance\n", what, bad, rows * cols);
   return bad;
 }
 
@@ -71,6 +72,39 @@ static std::vector<double> probe(const bri17::Hooke<double, DIM> &hooke) {
     u[j] = 1.0;
     K.apply_host(u.data(), Ku.data());
     for (int i = 0; i < n; i++) M[(size_t)i * n + j] = Ku[i];
+    u[j] = 0.0;
+  }
+  return M;
+}
+
+// tests/test_bri17.cpp:295-316
+template <int DIM>
+static std::vector<double> assemble_strain(const bri17::CartesianGrid<double, DIM> &grid,
+                                           const std::vector<double> &Be) {
+  const int nn = grid.num_nodes_per_cell, ne = nn * DIM, S = DIM * (DIM + 1) / 2;
+  const int rows = S * grid.size, cols = DIM * grid.size;
+  std::vector<double> B((size_t)rows * cols, 0.0);
+  for (int cell = 0; cell < grid.size; cell++) {
+    auto nodes = grid.get_cell_nodes(cell);
+    for (int il = 0; il < S; il++)
+      for (int je = 0; je < ne; je++) {
+        int j = nodes[je % nn] + grid.size * (je / nn);
+        B[(size_t)(il * grid.size + cell) * cols + j] += Be[(size_t)il * ne + je];
+      }
+  }
+  return B;
+}
+
+// StrainDisplacementMatrixFactory::run (tests/test_bri17.cpp:263-285) on the device
+template <int DIM>
+static std::vector<double> probe_strain(const bri17::Hooke<double, DIM> &hooke) {
+  bri17::StiffnessMatrix<double, DIM> K{hooke};
+  const int n = (int)K.num_dofs(), rows = DIM * (DIM + 1) / 2 * hooke.grid.size;
+  std::vector<double> u(n, 0.0), eps(rows), M((size_t)rows * n);
+  for (int j = 0; j < n; j++) {
+    u[j] = 1.0;
+    K.strain_host(u.data(), eps.data());
+    for (int i = 0; i < rows; i++) M[(size_t)i * n + j] = eps[i];
     u[j] = 0.0;
   }
   return M;
@@ -123,12 +157,27 @@ int main(int argc, char **argv) {
   {
     auto K_exp = assemble<2>(grid2, read_values(dir + "/Ke2.txt"));
     auto K_act = probe<2>(hooke2);
-    bad += assert_equal(K_exp, K_act, 2 * grid2.size, 1e-15, 1e-14, "2D stiffness matrix");
+    bad += assert_equal(K_exp, K_act, 2 * grid2.size, 2 * grid2.size, 1e-15, 1e-14,
+                        "2D stiffness matrix");
   }
   {
     auto K_exp = assemble<3>(grid3, read_values(dir + "/Ke3.txt"));
     auto K_act = probe<3>(hooke3);
-    bad += assert_equal(K_exp, K_act, 3 * grid3.size, 1e-15, 1e-14, "3D stiffness matrix");
+    bad += assert_equal(K_exp, K_act, 3 * grid3.size, 3 * grid3.size, 1e-15, 1e-14,
+                        "3D stiffness matrix");
+  }
+  // tests/test_bri17.cpp:538-605 (rtol 1e-15, atol 1e-14, :558,604)
+  {
+    auto B_exp = assemble_strain<2>(grid2, read_values(dir + "/Be2.txt"));
+    auto B_act = probe_strain<2>(hooke2);
+    bad += assert_equal(B_exp, B_act, 3 * grid2.size, 2 * grid2.size, 1e-15, 1e-14,
+                        "2D strain-displacement matrix");
+  }
+  {
+    auto B_exp = assemble_strain<3>(grid3, read_values(dir + "/Be3.txt"));
+    auto B_act = probe_strain<3>(hooke3);
+    bad += assert_equal(B_exp, B_act, 6 * grid3.size, 3 * grid3.size, 1e-15, 1e-14,
+                        "3D strain-displacement matrix");
   }
   return bad ? 1 : 0;
 }

@@ -44,7 +44,7 @@ def test_header_grid_and_per_frequency_functions(oracle):
 
 @pytest.mark.gpu
 def test_header_reference_assembly_sections_on_device(device):
-    """2D and 3D stiffness SECTIONs of the reference test through
+    """2D and 3D stiffness and strain SECTIONs of the reference test through
     bri17::StiffnessMatrix<double, DIM> on the GPU, at rtol 1e-15 / atol 1e-14."""
     r = _run("device")
     print(r.stdout)
