@@ -197,8 +197,10 @@ int bri17_autotune(int dim, const int* shape, const double* L, int dtype, int re
     }
   };
   consider(cur);  // automatic choices (fixed-size kernels where they exist)
-  // fixed-size path: persistent plane variants
-  for (int pf : {0, 1, 2, 3}) {
+  // fixed-size path: persistent plane variants (bits 0, 1); all-components-in-
+  // LDS pencil instead of the register-block one (bit 3) and its persistent
+  // variant (bit 2)
+  for (int pf : {0, 1, 2, 3, 8, 9, 12, 13}) {
     bri17_tuning t = cur;
     t.plane_prefetch = pf;
     consider(t);

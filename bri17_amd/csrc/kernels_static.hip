@@ -331,7 +331,22 @@ __global__ void __launch_bounds__(Q * M / R1)
 // thread (c, w, l) owns i0 = l + k TPA (k < A0) of sequence (c, w) for every
 // p: it performs the split DFT over p and the first axis-0 pass in registers.
 
-template <typename T, int N0, int A0, int B0, int P, int W, int MODE>
+template <typename T, int N0, int A0, int P, int W>
+__device__ __forceinline__ void pencil_load(cpx<T> (&v)[P][A0], const cpx<T>* __restrict__ ws,
+                                            int F, int c, int l, int fw) {
+  const cpx<T>* src = ws + (int64_t)c * N0 * P * F + fw;
+  constexpr int TPA = N0 / A0;
+#pragma unroll
+  for (int k = 0; k < A0; k++)
+#pragma unroll
+    for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = src[((int64_t)(l + k * TPA) * P + pp_) * F];
+}
+
+// Persistent variant (PF): one workgroup per CU slot walks its tiles, and the
+// loads of its next tile are issued into the registers freed by the first
+// axis-0 pass, so they are in flight while the rest of the tile is computed.
+// Tiles stay XCD-local: XCD x works through a contiguous range of tiles.
+template <typename T, int N0, int A0, int B0, int P, int W, int MODE, bool PF>
 __global__ void __launch_bounds__(3 * W * (N0 / A0))
     k_pencil_s(cpx<T>* __restrict__ ws, const PencilParams pp) {
   constexpr int TPA = N0 / A0;
@@ -346,8 +361,281 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
   const ModalCoef<T>* coef1 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]);
   const ModalCoef<T>* coef2 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[2]);
   const int tid = threadIdx.x;
+  const int F = pp.f;
+  const int T_ = pp.ntiles;
+
+  // tiles of this workgroup: first + j * stride < cnt, offset by base
+  int base = 0, cnt = T_, first = blockIdx.x, stride = gridDim.x;
+  if (pp.xcd_remap) {
+    // consecutive tiles share L2 lines: keep them on one XCD (workgroups are
+    // dealt round-robin over the 8 XCDs)
+    const int xx = blockIdx.x % 8, per = T_ / 8, rem = T_ % 8;
+    base = xx * per + (xx < rem ? xx : rem);
+    cnt = per + (xx < rem ? 1 : 0);
+    first = blockIdx.x / 8;
+    stride = PF ? (int)gridDim.x / 8 : cnt;  // one tile per workgroup without PF
+    if (PF && gridDim.x % 8) {
+      base = 0, cnt = T_, first = blockIdx.x, stride = gridDim.x;
+    }
+  }
+  if (first >= cnt) return;
+
+  copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, NT);
+
+  auto column = [&](int tile, int& f0, int& wv, int& fw) {
+    f0 = tile * W;
+    wv = F - f0 < W ? F - f0 : W;
+    const int w = tid % W;
+    // out-of-range columns of the last tile read a valid address and are never stored
+    fw = w < wv ? f0 + w : F - 1;
+  };
+
+  cpx<T> v[P][A0];
+  {
+    int f0, wv, fw;
+    column(base + first, f0, wv, fw);
+    pencil_load<T, N0, A0, P, W>(v, ws, F, tid / (W * TPA), (tid / W) % TPA, fw);
+  }
+  for (int j = first; j < cnt; j += stride) {
+    const int tl = opaque(tid);
+    const int t = base + j;
+    const int w = tl % W;
+    const int l = (tl / W) % TPA;
+    const int c = tl / (W * TPA);
+    int f0, wv, fw;
+    column(t, f0, wv, fw);
+    const bool wok = w < wv;
+    if (PF && j > first) __syncthreads();  // the previous tile's last LDS reads are done
+
+    // split DFT over p + pass 1
+    if constexpr (P > 1) {
+#pragma unroll
+      for (int k = 0; k < A0; k++) {
+        cpx<T> col[P];
+#pragma unroll
+        for (int pp_ = 0; pp_ < P; pp_++) col[pp_] = v[pp_][k];
+        bfly<P, false>(col);
+#pragma unroll
+        for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = col[pp_];
+      }
+    }
+#pragma unroll
+    for (int kb = 0; kb < P; kb++) {
+      bfly<A0, false>(v[kb]);
+#pragma unroll
+      for (int k = 0; k < A0; k++) buf[pd(((c * P + kb) * N0 + l * A0 + k) * W + w)] = v[kb][k];
+    }
+    // prefetch the next tile into the now free registers
+    if (PF && j + stride < cnt) {
+      int nf0, nwv, nfw;
+      column(t + stride, nf0, nwv, nfw);
+      pencil_load<T, N0, A0, P, W>(v, ws, F, c, l, nfw);
+    }
+    __syncthreads();
+    // pass 2 -> natural order k0 = jj + A0 k
+    {
+      cpx<T> y[P][A0 / B0][B0];
+#pragma unroll
+      for (int kb = 0; kb < P; kb++)
+#pragma unroll
+        for (int b = 0; b < A0 / B0; b++) {
+          const int jj = l + b * TPA;
+#pragma unroll
+          for (int k = 0; k < B0; k++) {
+            y[kb][b][k] = buf[pd(((c * P + kb) * N0 + jj + k * A0) * W + w)];
+            if (k) y[kb][b][k] = cmul(y[kb][b][k], tw[jj * k]);
+          }
+          bfly<B0, false>(y[kb][b]);
+        }
+      __syncthreads();
+#pragma unroll
+      for (int kb = 0; kb < P; kb++)
+#pragma unroll
+        for (int b = 0; b < A0 / B0; b++)
+#pragma unroll
+          for (int k = 0; k < B0; k++)
+            buf[pd(((c * P + kb) * N0 + l + b * TPA + k * A0) * W + w)] = y[kb][b][k];
+    }
+    __syncthreads();
+
+    // modal block: a thread keeps (kb, w) fixed and walks k0, so the factors
+    // of axes 1 and 2 are formed once
+    {
+      constexpr int KT = NT / (P * W);  // threads per (kb, w)
+      const int bw = tl % W;
+      const int bk = (tl / W) % KT;
+      const int bkb = tl / (W * KT);
+      const int f = f0 + bw;
+      const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
+      const int kl = f - ka * pp.rs;
+      if (bw < wv && kl < pp.nlh) {
+        const int k1 = ka + pp.q * bkb;
+        const T mu = T(pp.mu), s = T(pp.s);
+        const ModalCoef<T> b = coef1[k1], cc = coef2[kl];
+        const T chichi = b.chi * cc.chi, phichi = b.phi * cc.chi, chiphi = b.chi * cc.phi;
+        const T s_psichi = s * b.psi * cc.chi, s_chipsi = s * b.chi * cc.psi;
+        const T s_psipsi = s * b.psi * cc.psi;
+        const bool kz = k1 == 0 && kl == 0;
+        for (int k0 = bk; k0 < N0; k0 += KT) {
+          const ModalCoef<T> a = coef0[k0];
+          const T H0 = a.phi * chichi, H1 = a.chi * phichi, H2 = a.chi * chiphi;
+          const T dg = mu * (H0 + H1 + H2);
+          cpx<T> xx[3];
+#pragma unroll
+          for (int q3 = 0; q3 < 3; q3++) xx[q3] = buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)];
+          sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
+                              a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
+#pragma unroll
+          for (int q3 = 0; q3 < 3; q3++) buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)] = xx[q3];
+        }
+      }
+    }
+    __syncthreads();
+
+    // inverse pass 1 (inputs k0 = l + k TPA)
+    {
+      cpx<T> z[P][A0];
+#pragma unroll
+      for (int kb = 0; kb < P; kb++) {
+#pragma unroll
+        for (int k = 0; k < A0; k++) z[kb][k] = buf[pd(((c * P + kb) * N0 + l + k * TPA) * W + w)];
+        bfly<A0, true>(z[kb]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kb = 0; kb < P; kb++)
+#pragma unroll
+        for (int k = 0; k < A0; k++) buf[pd(((c * P + kb) * N0 + l * A0 + k) * W + w)] = z[kb][k];
+    }
+    __syncthreads();
+    // inverse pass 2 -> i0 = jj + A0 k, then the inverse split DFT and store
+    {
+      cpx<T> y[P][A0 / B0][B0];
+#pragma unroll
+      for (int kb = 0; kb < P; kb++)
+#pragma unroll
+        for (int b = 0; b < A0 / B0; b++) {
+          const int jj = l + b * TPA;
+#pragma unroll
+          for (int k = 0; k < B0; k++) {
+            y[kb][b][k] = buf[pd(((c * P + kb) * N0 + jj + k * A0) * W + w)];
+            if (k) y[kb][b][k] = cmulc(y[kb][b][k], tw[jj * k]);
+          }
+          bfly<B0, true>(y[kb][b]);
+        }
+      if constexpr (P > 1) {
+#pragma unroll
+        for (int b = 0; b < A0 / B0; b++)
+#pragma unroll
+          for (int k = 0; k < B0; k++) {
+            cpx<T> col[P];
+#pragma unroll
+            for (int kb = 0; kb < P; kb++) col[kb] = y[kb][b][k];
+            bfly<P, true>(col);
+#pragma unroll
+            for (int kb = 0; kb < P; kb++) y[kb][b][k] = col[kb];
+          }
+      }
+      if (wok) {
+        cpx<T>* dst = ws + (int64_t)c * N0 * P * F + f0 + w;
+#pragma unroll
+        for (int b = 0; b < A0 / B0; b++)
+#pragma unroll
+          for (int k = 0; k < B0; k++) {
+            const int i0 = l + b * TPA + k * A0;
+#pragma unroll
+            for (int pp_ = 0; pp_ < P; pp_++) dst[((int64_t)i0 * P + pp_) * F] = y[pp_][b][k];
+          }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------- pencil, N0 = R^2
+//
+// For N0 = R x R the axis-0 transform is one radix-R pass in registers, an
+// LDS transposition, and a second radix-R pass, and the output of the second
+// pass holds k0 = l + R j: exactly the input set of the inverse's first pass.
+// The components are therefore transformed one after the other through a
+// one-component LDS buffer, the modal block is applied in registers (a thread
+// holds all DIM components of its frequencies), and each component is
+// transformed back and stored.  LDS per tile is one component (P N0 W
+// entries), so W = 16 tiles (128-byte rows) fit twice per CU.  The split
+// index p is a lane bit (partner lane = lane ^ W): the P = 2 split DFT is a
+// lane exchange.
+
+template <typename T, int R, int W, bool INV>
+__device__ __forceinline__ void pencil_r_fft(cpx<T> (&v)[R], cpx<T>* buf, const cpx<T>* tw,
+                                             int rowbase, int l, int w) {
+  // registers (index k of n = l + R k) -> radix R -> LDS [p][m][l] -> twiddle
+  // w_N0^(i l') -> radix R -> registers (index j of n' = l' + R j)
+  bfly<R, INV>(v);
+#pragma unroll
+  for (int m = 0; m < R; m++) buf[pd((rowbase + m) * R * W + l * W + w)] = v[m];
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    cpx<T> x = buf[pd((rowbase + l) * R * W + i * W + w)];
+    if (i) x = INV ? cmulc(x, tw[i * l]) : cmul(x, tw[i * l]);
+    v[i] = x;
+  }
+  bfly<R, INV>(v);
+  __syncthreads();  // the buffer is reused by the next component
+}
+
+template <typename T>
+__device__ __forceinline__ cpx<T> shfl_xor_c(cpx<T> x, int m) {
+  return cpx<T>{__shfl_xor(x.x, m), __shfl_xor(x.y, m)};
+}
+
+// P-point DFT (P = 2 or 4) over the split index held by the P lanes p W + w
+// of a group.  Forward: lane p holds x_p on entry and y_kb on exit, kb the
+// bit-reversed p (radix-2 butterfly over the high bit of p, twiddle, radix-2
+// over the low bit); inverse: the conjugate transposed network, from the
+// bit-reversed spectrum back to x_p at lane p (unnormalised).
+template <typename T, int P, int W, bool INV>
+__device__ __forceinline__ cpx<T> split_dft(cpx<T> x, int p) {
+  auto bfly_lanes = [&](cpx<T> v, int m, bool second) {
+    const cpx<T> o = shfl_xor_c(v, m);
+    return second ? cpx<T>{o.x - v.x, o.y - v.y} : cpx<T>{v.x + o.x, v.y + o.y};
+  };
+  if constexpr (P == 2) {
+    return bfly_lanes(x, W, p & 1);
+  } else {
+    static_assert(P == 4, "split of 2 or 4");
+    const bool tw = p == 3;  // w_4^(p0 q0) with p0 = q0 = 1
+    if (!INV) {
+      x = bfly_lanes(x, 2 * W, p & 2);
+      if (tw) x = cpx<T>{x.y, -x.x};  // times -i
+      return bfly_lanes(x, W, p & 1);
+    } else {
+      x = bfly_lanes(x, W, p & 1);
+      if (tw) x = cpx<T>{-x.y, x.x};  // times +i
+      return bfly_lanes(x, 2 * W, p & 2);
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ int split_bitrev(int p) {
+  return P == 4 ? ((p & 1) << 1) | (p >> 1) : p;
+}
+
+template <typename T, int R, int P, int W, int MODE>
+__global__ void __launch_bounds__(W * R * P)
+    k_pencil_r(cpx<T>* __restrict__ ws, const PencilParams pp, int ntiles) {
+  constexpr int N0 = R * R;
+  static_assert(P * W <= 64, "the split group must lie in one wavefront");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
+  cpx<T>* tw = buf + pd_size(P * N0 * W);
+  const ModalCoef<T>* coef0 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[0]);
+  const ModalCoef<T>* coef1 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]);
+  const ModalCoef<T>* coef2 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[2]);
+  const int tid = threadIdx.x;
   int t = blockIdx.x;
   if (pp.xcd_remap) {
+    // consecutive tiles share L2 lines: keep them on one XCD
     const int n = gridDim.x, per = n / 8, rem = n % 8;
     const int xx = t % 8, idx = t / 8;
     t = xx * per + (xx < rem ? xx : rem) + idx;
@@ -356,152 +644,74 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
   const int f0 = t * W;
   const int wv = F - f0 < W ? F - f0 : W;
   const int w = tid % W;
-  const int l = (tid / W) % TPA;
-  const int c = tid / (W * TPA);
+  const int p = (tid / W) % P;
+  const int l = tid / (W * P);
   const bool wok = w < wv;
-  // out-of-range columns of the last tile read a valid address and are never stored
-  const int fw = wok ? f0 + w : F - 1;
+  const int kb = split_bitrev<P>(p);  // split frequency this lane holds after the split DFT
+  copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, W * R * P);
+  // (c, i0 = l + R k, p) lies at (l P + p) F + f + (c N0 + R k) P F.  The
+  // lane's pointer is re-derived from an opaque copy of the thread index per
+  // component, so the compiler does not precompute (and keep) every address.
+  auto lane_ptr = [&](int c) {
+    const int tl = opaque(tid);
+    const int ll = tl / (W * P), lp = (tl / W) % P;
+    const int lw = tl % W;
+    // out-of-range columns of the last tile read a valid address and are never stored
+    const int lf = lw < wv ? f0 + lw : F - 1;
+    return ws + (int64_t)(ll * P + lp) * F + lf + (int64_t)c * N0 * P * F;
+  };
 
-  copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, NT);
+  cpx<T> X[3][R];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const cpx<T>* src = lane_ptr(c);
+#pragma unroll
+    for (int k = 0; k < R; k++) X[c][k] = src[(int64_t)(R * k) * P * F];
+#pragma unroll
+    for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, false>(X[c][k], p);
+    pencil_r_fft<T, R, W, false>(X[c], buf, tw, kb * R, l, w);  // (its barrier also covers tw)
+  }
 
-  // load + split DFT over p + pass 1
-  cpx<T> v[P][A0];
+  // modal block at k0 = l + R j, k1 = ka + Q kb
   {
-    const cpx<T>* src = ws + (int64_t)c * N0 * P * F + fw;
-#pragma unroll
-    for (int k = 0; k < A0; k++)
-#pragma unroll
-      for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = src[((int64_t)(l + k * TPA) * P + pp_) * F];
-  }
-  if constexpr (P > 1) {
-#pragma unroll
-    for (int k = 0; k < A0; k++) {
-      cpx<T> col[P];
-#pragma unroll
-      for (int pp_ = 0; pp_ < P; pp_++) col[pp_] = v[pp_][k];
-      bfly<P, false>(col);
-#pragma unroll
-      for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = col[pp_];
-    }
-  }
-#pragma unroll
-  for (int kb = 0; kb < P; kb++) {
-    bfly<A0, false>(v[kb]);
-#pragma unroll
-    for (int k = 0; k < A0; k++) buf[pd(((c * P + kb) * N0 + l * A0 + k) * W + w)] = v[kb][k];
-  }
-  __syncthreads();
-  // pass 2 -> natural order k0 = jj + A0 k
-  {
-    cpx<T> y[P][A0 / B0][B0];
-#pragma unroll
-    for (int kb = 0; kb < P; kb++)
-#pragma unroll
-      for (int b = 0; b < A0 / B0; b++) {
-        const int jj = l + b * TPA;
-#pragma unroll
-        for (int k = 0; k < B0; k++) {
-          y[kb][b][k] = buf[pd(((c * P + kb) * N0 + jj + k * A0) * W + w)];
-          if (k) y[kb][b][k] = cmul(y[kb][b][k], tw[jj * k]);
-        }
-        bfly<B0, false>(y[kb][b]);
-      }
-    __syncthreads();
-#pragma unroll
-    for (int kb = 0; kb < P; kb++)
-#pragma unroll
-      for (int b = 0; b < A0 / B0; b++)
-#pragma unroll
-        for (int k = 0; k < B0; k++)
-          buf[pd(((c * P + kb) * N0 + l + b * TPA + k * A0) * W + w)] = y[kb][b][k];
-  }
-  __syncthreads();
-
-  // modal block: a thread keeps (kb, w) fixed and walks k0, so the factors of
-  // axes 1 and 2 are formed once
-  {
-    constexpr int KT = NT / (P * W);  // threads per (kb, w)
-    const int bw = tid % W;
-    const int bk = (tid / W) % KT;
-    const int bkb = tid / (W * KT);
-    const int f = f0 + bw;
+    const int f = f0 + w;
     const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
     const int kl = f - ka * pp.rs;
-    if (bw < wv && kl < pp.nlh) {
-      const int k1 = ka + pp.q * bkb;
+    if (wok && kl < pp.nlh) {
       const T mu = T(pp.mu), s = T(pp.s);
-      const ModalCoef<T> b = coef1[k1], cc = coef2[kl];
+      const ModalCoef<T> cc = coef2[kl];
+      const int k1 = ka + pp.q * kb;
+      const ModalCoef<T> b = coef1[k1];
       const T chichi = b.chi * cc.chi, phichi = b.phi * cc.chi, chiphi = b.chi * cc.phi;
       const T s_psichi = s * b.psi * cc.chi, s_chipsi = s * b.chi * cc.psi;
       const T s_psipsi = s * b.psi * cc.psi;
       const bool kz = k1 == 0 && kl == 0;
-      for (int k0 = bk; k0 < N0; k0 += KT) {
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int k0 = l + R * j;
         const ModalCoef<T> a = coef0[k0];
         const T H0 = a.phi * chichi, H1 = a.chi * phichi, H2 = a.chi * chiphi;
         const T dg = mu * (H0 + H1 + H2);
-        cpx<T> xx[3];
-#pragma unroll
-        for (int q3 = 0; q3 < 3; q3++) xx[q3] = buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)];
+        cpx<T> xx[3] = {X[0][j], X[1][j], X[2][j]};
         sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
                             a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
-#pragma unroll
-        for (int q3 = 0; q3 < 3; q3++) buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)] = xx[q3];
+        X[0][j] = xx[0];
+        X[1][j] = xx[1];
+        X[2][j] = xx[2];
       }
     }
   }
-  __syncthreads();
 
-  // inverse pass 1 (inputs k0 = l + k TPA)
+  // inverse: per component, the two passes, the inverse split DFT, the store
 #pragma unroll
-  for (int kb = 0; kb < P; kb++) {
+  for (int c = 0; c < 3; c++) {
+    pencil_r_fft<T, R, W, true>(X[c], buf, tw, kb * R, l, w);
 #pragma unroll
-    for (int k = 0; k < A0; k++) v[kb][k] = buf[pd(((c * P + kb) * N0 + l + k * TPA) * W + w)];
-    bfly<A0, true>(v[kb]);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int kb = 0; kb < P; kb++)
-#pragma unroll
-    for (int k = 0; k < A0; k++) buf[pd(((c * P + kb) * N0 + l * A0 + k) * W + w)] = v[kb][k];
-  __syncthreads();
-  // inverse pass 2 -> i0 = jj + A0 k, then the inverse split DFT and store
-  {
-    cpx<T> y[P][A0 / B0][B0];
-#pragma unroll
-    for (int kb = 0; kb < P; kb++)
-#pragma unroll
-      for (int b = 0; b < A0 / B0; b++) {
-        const int jj = l + b * TPA;
-#pragma unroll
-        for (int k = 0; k < B0; k++) {
-          y[kb][b][k] = buf[pd(((c * P + kb) * N0 + jj + k * A0) * W + w)];
-          if (k) y[kb][b][k] = cmulc(y[kb][b][k], tw[jj * k]);
-        }
-        bfly<B0, true>(y[kb][b]);
-      }
-    if constexpr (P > 1) {
-#pragma unroll
-      for (int b = 0; b < A0 / B0; b++)
-#pragma unroll
-        for (int k = 0; k < B0; k++) {
-          cpx<T> col[P];
-#pragma unroll
-          for (int kb = 0; kb < P; kb++) col[kb] = y[kb][b][k];
-          bfly<P, true>(col);
-#pragma unroll
-          for (int kb = 0; kb < P; kb++) y[kb][b][k] = col[kb];
-        }
-    }
+    for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, true>(X[c][k], p);
     if (wok) {
-      cpx<T>* dst = ws + (int64_t)c * N0 * P * F + f0 + w;
+      cpx<T>* dst = lane_ptr(c);
 #pragma unroll
-      for (int b = 0; b < A0 / B0; b++)
-#pragma unroll
-        for (int k = 0; k < B0; k++) {
-          const int i0 = l + b * TPA + k * A0;
-#pragma unroll
-          for (int pp_ = 0; pp_ < P; pp_++) dst[((int64_t)i0 * P + pp_) * F] = y[pp_][b][k];
-        }
+      for (int k = 0; k < R; k++) dst[(int64_t)(R * k) * P * F] = X[c][k];
     }
   }
 }
@@ -528,6 +738,36 @@ struct PencilS {
   static size_t lds() { return (pd_size(3 * P * N0 * W) + N0) * sizeof(cpx<T>); }
   static constexpr int threads() { return 3 * W * (N0 / A0); }
 };
+
+// Pencil launch of a fixed-size case: the register-block pencil for N0 = R^2
+// (fp32, P = 2 or 4) unless bit 3 of `pf` asks for the all-components-in-LDS
+// pencil; bit 2: persistent variant of the latter.
+template <typename T, int N0, int A0, int B0, int P, int W>
+static hipError_t launch_pencil_static(cpx<T>* ws, const PencilParams& pc, int mode, int pf,
+                                       hipStream_t st) {
+  if constexpr (A0 == B0 && A0 * B0 == N0 && (P == 2 || P == 4) && sizeof(T) == 4) {
+    if (!(pf & 8)) {
+      constexpr int WR = 32 / P;  // 128-byte rows for P = 2
+      auto kp = mode == 0 ? k_pencil_r<T, A0, P, WR, 0> : k_pencil_r<T, A0, P, WR, 1>;
+      const size_t lds = (pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>);
+      hipError_t e = set_lds_s(kp, lds);
+      if (e != hipSuccess) return e;
+      const int nt = (pc.f + WR - 1) / WR;
+      hipLaunchKernelGGL(kp, dim3(nt), dim3(WR * A0 * P), lds, st, ws, pc, nt);
+      return hipGetLastError();
+    }
+  }
+  using KS = PencilS<T, N0, A0, B0, P, W>;
+  auto kp = mode == 0 ? ((pf & 4) ? k_pencil_s<T, N0, A0, B0, P, W, 0, true>
+                                  : k_pencil_s<T, N0, A0, B0, P, W, 0, false>)
+                      : ((pf & 4) ? k_pencil_s<T, N0, A0, B0, P, W, 1, true>
+                                  : k_pencil_s<T, N0, A0, B0, P, W, 1, false>);
+  hipError_t e = set_lds_s(kp, KS::lds());
+  if (e != hipSuccess) return e;
+  const int gp = (pf & 4) ? resident_blocks(kp, KS::threads(), KS::lds(), pc.ntiles) : pc.ntiles;
+  hipLaunchKernelGGL(kp, dim3(gp), dim3(KS::threads()), KS::lds(), st, ws, pc);
+  return hipGetLastError();
+}
 
 // Supported fixed-size geometries:
 //   dtype, N0, N1, N2,
@@ -587,15 +827,12 @@ hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void
 #define B17_RUN(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                                   \
   if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                       \
     using PS = PlaneS<T, M, R1, R2, Q, C1, C2, P>;                                                \
-    using KS = PencilS<T, A, A0, B0, P, W>;                                                       \
     auto kf = (pf & 1) ? k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, true>                         \
                        : k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, false>;                        \
     auto ki = (pf & 2) ? k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, true>                         \
                        : k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, false>;                        \
-    auto kp = mode == 0 ? k_pencil_s<T, A, A0, B0, P, W, 0> : k_pencil_s<T, A, A0, B0, P, W, 1>;  \
     e = set_lds_s(kf, PS::lds_fwd());                                                             \
     if (e == hipSuccess) e = set_lds_s(ki, PS::lds_inv());                                        \
-    if (e == hipSuccess) e = set_lds_s(kp, KS::lds());                                            \
     const int nitems = 3 * A * P;                                                                 \
     const int gf = (pf & 1) ? resident_blocks(kf, PS::threads(), PS::lds_fwd(), nitems) : nitems; \
     const int gi = (pf & 2) ? resident_blocks(ki, PS::threads(), PS::lds_inv(), nitems) : nitems; \
@@ -607,8 +844,7 @@ hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void
     }                                                                                             \
     mark(1);                                                                                      \
     if (e == hipSuccess) {                                                                        \
-      hipLaunchKernelGGL(kp, dim3(pc.ntiles), dim3(KS::threads()), KS::lds(), st, (cpx<T>*)ws, pc); \
-      e = hipGetLastError();                                                                      \
+      e = launch_pencil_static<T, A, A0, B0, P, W>((cpx<T>*)ws, pc, mode, pf, st);               \
     }                                                                                             \
     mark(2);                                                                                      \
     if (e == hipSuccess) {                                                                        \

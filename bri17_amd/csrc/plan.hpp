@@ -29,7 +29,7 @@ struct Tuning {
   size_t lds_budget = 160 * 1024;
   int xcd_remap = 1;
   int force_generic = 0;        // 1: never use the fixed-size kernels
-  int plane_prefetch = 1;       // fixed-size plane kernels: bit0 fwd, bit1 inv persistent
+  int plane_prefetch = 1;       // fixed-size kernels persistent: bit0 fwd, bit1 inv, bit2 pencil
   std::vector<int> radices[3];  // forced radix orders per FFT length role
 };
 

@@ -102,9 +102,12 @@ typedef struct bri17_tuning {
                          transforms, 0-terminated (radices 2,3,4,5,7,8,16) */
   int force_generic;  /* 1: use the runtime-size kernels even where a
                          fixed-size instantiation exists */
-  int plane_prefetch; /* fixed-size plane kernels run persistently with the
-                         next plane's loads in flight: bit 0 forward, bit 1
-                         inverse; -1 = automatic */
+  int plane_prefetch; /* fixed-size kernels: bits 0 / 1 run the forward /
+                         inverse plane kernel persistently with the next work
+                         item's loads in flight; bit 3 selects the
+                         all-components-in-LDS pencil instead of the
+                         register-block one (N0 = R^2 grids), bit 2 its
+                         persistent variant; -1 = automatic */
 } bri17_tuning;
 
 /* bri17_operator_create with explicit tuning (NULL = automatic). */
