@@ -817,6 +817,24 @@ bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g) {
   return false;
 }
 
+std::string static_describe(int dtype, int n0, int n1, int n2, int pf) {
+#define B17_DESC(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                                  \
+  if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                       \
+    const bool reg = A0 == B0 && A0 * B0 == A && (P == 2 || P == 4) && sizeof(T) == 4 && !(pf & 8); \
+    const int wr = reg ? 32 / P : W;                                                              \
+    return std::string("{\"plane_split\":") + std::to_string(P) +                                 \
+           ",\"plane_threads\":" + std::to_string(Q * M / R1) +                                  \
+           ",\"plane_forward_persistent\":" + ((pf & 1) ? "true" : "false") +                   \
+           ",\"plane_inverse_persistent\":" + ((pf & 2) ? "true" : "false") +                   \
+           ",\"pencil\":\"" + (reg ? "register-block" : "lds") + "\",\"pencil_width\":" +       \
+           std::to_string(wr) + ",\"pencil_threads\":" +                                        \
+           std::to_string(reg ? wr * A0 * P : 3 * W * (A / A0)) + "}";                            \
+  }
+  B17_STATIC_CASES(B17_DESC)
+#undef B17_DESC
+  return std::string();
+}
+
 hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void* in, void* out,
                          void* ws, const PencilParams& pc, const void* twr, const void* twc,
                          int rs, int pf, hipStream_t st, hipEvent_t* ev) {

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "kernels.hpp"
 
 namespace b17 {
@@ -17,6 +19,10 @@ struct StaticGeom {
 
 // True if (dtype, N0, N1, N2) has a fixed-size instantiation; fills g.
 bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g);
+
+// JSON object describing the fixed-size kernels a plan launches (empty if
+// the grid has none).
+std::string static_describe(int dtype, int n0, int n1, int n2, int pf);
 
 // Launch the three fixed-size kernels (ev: optional 4 timing events).
 // pf bit 0 / bit 1: persistent prefetching forward / inverse plane kernel.

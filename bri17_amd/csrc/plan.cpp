@@ -386,7 +386,11 @@ std::string describe_plan(const Plan& pl) {
     << ",\"threads\":" << pl.pencil_threads << ",\"lds\":" << pl.pencil_lds
     << ",\"xcd_remap\":" << pc.xcd_remap << ",\"axis0_fft\":{\"n\":" << pc.ax0.n
     << ",\"tpt\":" << pc.ax0.tpt << ",\"radices\":" << radices_json(pc.ax0) << "}}"
-    << ",\"workspace_bytes\":" << pl.ws_bytes << "}";
+    << ",\"workspace_bytes\":" << pl.ws_bytes;
+  if (pl.use_static)
+    o << ",\"fixed_size\":"
+      << static_describe(pl.dtype, pl.shape[0], pl.shape[1], pl.shape[2], pl.tuning.plane_prefetch);
+  o << "}";
   return o.str();
 }
 

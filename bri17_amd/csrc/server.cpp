@@ -223,7 +223,10 @@ int launch_wave(bri17_server* s, Slot& sl) {
       return BRI17_ERR_HIP;
   auto in_of = [](const Req& q) { return q.r.memory == BRI17_MEM_HOST ? q.d_in : q.r.in; };
   auto out_of = [](const Req& q) { return q.r.memory == BRI17_MEM_HOST ? q.d_out : q.r.out; };
-  if (sl.batched) {
+  // A wave of one batchable request runs the plan's own three kernels: the
+  // same runtime-size arithmetic as the persistent launch (so the same bits),
+  // without its table upload and dependency counters.
+  if (sl.batched && sl.reqs.size() > 1) {
     std::vector<const b17::Plan*> plans;
     for (auto& q : sl.reqs) plans.push_back(q->plan.get());
     int threads = 0;
@@ -260,7 +263,8 @@ int launch_wave(bri17_server* s, Slot& sl) {
 
 void finish_wave(bri17_server* s, Slot& sl, int code) {
   const Clock::time_point t = Clock::now();
-  if (!code && sl.batched && *sl.h_status) code = BRI17_ERR_STATE;  // a dependency wait timed out
+  if (!code && sl.batched && sl.reqs.size() > 1 && *sl.h_status)
+    code = BRI17_ERR_STATE;  // a dependency wait timed out
   for (auto& q : sl.reqs) unadmit(s, *q);
   {
     std::lock_guard<std::mutex> g(s->mtx);

@@ -177,6 +177,21 @@ def test_config3_3d_256_fp32(device, oracle):
     assert e1 <= 1e-5 and e2 <= 1e-5
 
 
+def test_fixed_size_plan_description(device):
+    """The description of a fixed-size plan names the kernels it launches: the
+    register-block pencil for 256^3 fp32 (N0 = 16^2), the LDS pencil when
+    asked for (bit 3 of plane_prefetch) and for the other fixed sizes."""
+    from bri17_amd import StiffnessMatrix
+    d = StiffnessMatrix((256, 256, 256), None, 1.0, 0.3, "float32").describe()
+    assert d["fixed_size_kernels"] and d["fixed_size"]["pencil"] == "register-block"
+    assert d["fixed_size"]["pencil_width"] == 16 and d["fixed_size"]["plane_split"] == 2
+    d = StiffnessMatrix((256, 256, 256), None, 1.0, 0.3, "float32", tuning=dict(plane_prefetch=9)).describe()
+    assert d["fixed_size"]["pencil"] == "lds"
+    d = StiffnessMatrix((128, 128, 128), None, 1.0, 0.3, "float32").describe()
+    assert d["fixed_size"]["pencil"] == "lds"
+    assert "fixed_size" not in StiffnessMatrix((48, 36, 54), None, 1.0, 0.3, "float32").describe()
+
+
 @pytest.mark.parametrize("split", [6, 9, 18, 27])
 @pytest.mark.parametrize("mode", ["apply", "solve"])
 def test_middle_axis_splits_without_fused_butterfly(device, oracle, split, mode):
