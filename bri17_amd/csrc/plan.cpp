@@ -224,21 +224,24 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
 
   if (dim == 3) {
     const int n1 = shape[1];
-    // Smallest split P of the middle axis whose Q = N1/P rows fit in LDS,
-    // preferably within kPlaneLds (several plane workgroups share a CU and
-    // the batch kernel's items stay small), leaving room for a pencil tile
-    // of width 1 (DIM N0 P complex entries).  Any divisor up to 64 is a
-    // candidate; splits without a fused butterfly run a Stockham P-point
-    // transform in the pencil.
+    // Smallest split P of the middle axis whose Q = N1/P rows fit in LDS.
+    // In order of preference: the plane within kPlaneLds (several plane
+    // workgroups share a CU and the batch kernel's items stay small) with
+    // pencil tiles of at least 8 columns (64-byte rows of DIM N0 P entries);
+    // any plane with such tiles; then narrower tiles, down to one column.
+    // Any divisor up to 64 is a candidate; splits without a fused butterfly
+    // run a Stockham P-point transform in the pencil.
     int P = 0;
     std::vector<int> cands = {1};
     for (int c = 2; c <= 64 && c <= n1; c++)
       if (n1 % c == 0 && !default_radices(c).empty()) cands.push_back(c);
-    for (size_t b : {std::min(budget, kPlaneLds), budget}) {
+    const size_t pencil_col = (size_t)dim * shape[0] * csz;  // one column, per unit of P
+    const struct { size_t plane; int min_w; } prefs[] = {
+        {std::min(budget, kPlaneLds), 8}, {budget, 8}, {std::min(budget, kPlaneLds), 1}, {budget, 1}};
+    for (const auto& pr : prefs) {
       for (int c : cands) {
         if (tun.plane_split && c != tun.plane_split) continue;
-        if (n1 % c) continue;
-        if ((size_t)(n1 / c) * rsl * csz <= b && (size_t)dim * shape[0] * c * csz <= budget) {
+        if ((size_t)(n1 / c) * rsl * csz <= pr.plane && pencil_col * c * pr.min_w <= budget) {
           P = c;
           break;
         }
