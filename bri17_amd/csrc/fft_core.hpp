@@ -1,5 +1,5 @@
 // Device-side FFT engine for gfx950: Stockham autosort passes on sequences
-// staged in LDS, radix 2/3/4/5/7/8/16 butterflies held in VGPRs.
+// staged in LDS, radix 2/3/4/5/7/8/9/16 butterflies held in VGPRs.
 //
 // Replaces the FFTW c2c transforms of the reference apply harness
 // (tests/test_bri17.cpp:57,95, plans :119-127).  Conventions follow FFTW /
@@ -177,6 +177,37 @@ __device__ __forceinline__ void bfly8(cpx<T>* v) {
   for (int i = 0; i < 8; i++) v[i] = y[i];
 }
 
+// radix 9 = 3 x 3: DFT3 over q of x[3q+p], twiddle w9^(p ka), DFT3 over p.
+template <bool INV, typename T>
+__device__ __forceinline__ void bfly9(cpx<T>* v) {
+  const T c1 = T(0.76604444311897803520239265055541667);   // cos(2 pi/9)
+  const T s1 = T(0.64278760968653932632264340990726344);   // sin(2 pi/9)
+  const T c2 = T(0.17364817766693034885171662676931480);   // cos(4 pi/9)
+  const T s2 = T(0.98480775301220805936674302458952301);   // sin(4 pi/9)
+  const T c4 = T(-0.93969262078590838405410927732473146);  // cos(8 pi/9)
+  const T s4 = T(0.34202014332566873304409961468225958);   // sin(8 pi/9)
+  cpx<T> y[3][3];  // y[p][ka]
+#pragma unroll
+  for (int p = 0; p < 3; p++) {
+    y[p][0] = v[p];
+    y[p][1] = v[p + 3];
+    y[p][2] = v[p + 6];
+    bfly3<INV>(y[p]);
+  }
+  y[1][1] = crot<INV>(y[1][1], c1, s1);  // e = 1
+  y[1][2] = crot<INV>(y[1][2], c2, s2);  // e = 2
+  y[2][1] = crot<INV>(y[2][1], c2, s2);  // e = 2
+  y[2][2] = crot<INV>(y[2][2], c4, s4);  // e = 4
+#pragma unroll
+  for (int ka = 0; ka < 3; ka++) {
+    cpx<T> z[3] = {y[0][ka], y[1][ka], y[2][ka]};
+    bfly3<INV>(z);
+    v[ka] = z[0];
+    v[ka + 3] = z[1];
+    v[ka + 6] = z[2];
+  }
+}
+
 // radix 16 = 4 x 4: DFT4 over q of x[4q+p], twiddle w16^(p ka), DFT4 over p.
 template <bool INV, typename T>
 __device__ __forceinline__ void bfly16(cpx<T>* v) {
@@ -223,6 +254,8 @@ __device__ __forceinline__ void bfly(cpx<T>* v) {
     bfly7<INV>(v);
   else if constexpr (R == 8)
     bfly8<INV>(v);
+  else if constexpr (R == 9)
+    bfly9<INV>(v);
   else if constexpr (R == 16)
     bfly16<INV>(v);
 }
@@ -318,6 +351,9 @@ __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
         break;
       case 8:
         fft_pass<T, 8, INV, MAXE / 8>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        break;
+      case 9:
+        fft_pass<T, 9, INV, MAXE / 9>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
         break;
       case 16:
         fft_pass<T, 16, INV, MAXE / 16>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);

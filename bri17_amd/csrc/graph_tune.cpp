@@ -234,12 +234,16 @@ int bri17_autotune(int dim, const int* shape, const double* L, int dtype, int re
     if (n < 4) continue;
     std::vector<std::vector<int>> orders;
     for (int r0 : {16, 8, 4, 2}) {
-      std::vector<int> o;
-      int m = n;
-      while (m % r0 == 0 && (int)o.size() < 7) o.push_back(r0), m /= r0;
-      for (int r : {8, 4, 2, 3, 5, 7})
-        while (m % r == 0 && (int)o.size() < 7) o.push_back(r), m /= r;
-      if (m == 1 && !o.empty()) orders.push_back(o);
+      for (int r3 : {9, 3}) {  // odd part in radix-9 or radix-3 passes
+        std::vector<int> o;
+        int m = n;
+        while (m % r0 == 0 && (int)o.size() < 7) o.push_back(r0), m /= r0;
+        for (int r : {8, 4, 2, r3, 3, 5, 7})
+          while (m % r == 0 && (int)o.size() < 7) o.push_back(r), m /= r;
+        if (m == 1 && !o.empty() &&
+            std::find(orders.begin(), orders.end(), o) == orders.end())
+          orders.push_back(o);
+      }
     }
     const size_t nfwd = orders.size();
     for (size_t i = 0; i < nfwd; i++) {

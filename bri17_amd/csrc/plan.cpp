@@ -35,6 +35,7 @@ std::vector<int> default_radices(int n) {
   if (n % 8 == 0) r.push_back(8), n /= 8;
   if (n % 4 == 0) r.push_back(4), n /= 4;
   if (n % 2 == 0) r.push_back(2), n /= 2;
+  while (n % 9 == 0) r.push_back(9), n /= 9;
   for (int p : {3, 5, 7})
     while (n % p == 0) r.push_back(p), n /= p;
   if (n != 1) r.clear();
@@ -53,7 +54,7 @@ bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
   }
   int prod = 1, rmax = 1;
   for (int r : radices) {
-    if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 16)) {
+    if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 9 || r == 16)) {
       if (why) *why = "unsupported radix " + std::to_string(r);
       return false;
     }

@@ -99,7 +99,7 @@ typedef struct bri17_tuning {
   int lds_budget_kb;  /* LDS per workgroup, KiB (<= 160) */
   int xcd_remap;      /* 0: off, 1: on, -1: automatic (on) */
   int radices[3][8];  /* radix order of the row / middle-axis / axis-0
-                         transforms, 0-terminated (radices 2,3,4,5,7,8,16) */
+                         transforms, 0-terminated (radices 2,3,4,5,7,8,9,16) */
   int force_generic;  /* 1: use the runtime-size kernels even where a
                          fixed-size instantiation exists */
   int plane_prefetch; /* fixed-size kernels: bits 0 / 1 run the forward /
