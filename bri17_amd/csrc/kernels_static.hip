@@ -74,6 +74,18 @@ __device__ __forceinline__ void plane_load_rows(cpx<T>* v, const T* __restrict__
   for (int k = 0; k < R1; k++) v[k] = src[j + k * TPR];
 }
 
+// LDS index of (row, col) in rows of RS entries padded by one entry every 32
+// (pd(row RS + col) for RS a multiple of 32).  Callers split it into a lane
+// part and a compile-time part so that the latter becomes an instruction
+// offset: row and col terms are independent, and col >> 5 splits when the
+// constant part of col is a multiple of 32 or cannot carry into bit 5.
+template <int RS>
+__device__ __forceinline__ constexpr int prow(int row) {
+  static_assert(RS % 32 == 0, "padded rows of 32-entry multiples");
+  return row * (RS + RS / 32);
+}
+__device__ __forceinline__ constexpr int pcol(int col) { return col + (col >> 5); }
+
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
 __global__ void __launch_bounds__(Q * M / R1)
     k_plane_fwd_s(const T* __restrict__ u, cpx<T>* __restrict__ ws, const cpx<T>* __restrict__ g_twr,
@@ -85,6 +97,7 @@ __global__ void __launch_bounds__(Q * M / R1)
   constexpr int RS = M;  // row stride (before padding)
   static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
+  static_assert(32 % R1 == 0 && M % 32 == 0, "LDS offsets split into lane + constant parts");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* twr = buf + pd_size(Q * RS);   // w_{2M}^j, j < 2M
@@ -105,22 +118,27 @@ __global__ void __launch_bounds__(Q * M / R1)
     const int cs = tl % M, l = tl / M;
     const int plane = item / P;
     const int p = item - plane * P;
-    // ---- rows: pass 1 on the loaded registers
-    bfly<R1, false>(v);
+    // ---- rows: pass 1 on the loaded registers; (q, j R1 + k), k < R1
+    {
+      bfly<R1, false>(v);
+      cpx<T>* bp = buf + prow<RS>(q) + pcol(j * R1);
 #pragma unroll
-    for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
+      for (int k = 0; k < R1; k++) bp[k] = v[k];
+    }
     // prefetch the next work item into the now free registers
     if (PF && item + (int)gridDim.x < nitems)
       plane_load_rows<T, M, R1, Q, P>(v, u, item + gridDim.x, tl);
     __syncthreads();
-    // pass 2: R1/R2 butterflies per thread, natural-order output
+    // pass 2: R1/R2 butterflies per thread, natural-order output;
+    // (q, jj + k R1) with jj = j + b TPR < R1
     cpx<T> w[R1 / R2][R2];
+    cpx<T>* rp = buf + prow<RS>(q) + j;
 #pragma unroll
     for (int b = 0; b < R1 / R2; b++) {
       const int jj = j + b * TPR;
 #pragma unroll
       for (int k = 0; k < R2; k++) {
-        w[b][k] = buf[pd(q * RS + jj + k * R1)];
+        w[b][k] = rp[b * TPR + pcol(k * R1)];
         if (k) w[b][k] = cmul(w[b][k], twr[2 * jj * k]);
       }
       bfly<R2, false>(w[b]);
@@ -129,38 +147,48 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
     for (int b = 0; b < R1 / R2; b++)
 #pragma unroll
-      for (int k = 0; k < R2; k++) buf[pd(q * RS + j + b * TPR + k * R1)] = w[b][k];
+      for (int k = 0; k < R2; k++) rp[b * TPR + pcol(k * R1)] = w[b][k];
     __syncthreads();
 
-    // ---- columns: pass 1 reads Z of two columns and forms the spectrum
+    // ---- columns: pass 1 reads Z of two columns and forms the spectrum;
+    // rows qq = l + kk TPC of columns cs and M - cs
     cpx<T> x[C1];
+    {
+      const cpx<T>* cp = buf + prow<RS>(l) + pcol(cs);
+      const cpx<T>* cm = buf + prow<RS>(l) + pcol(M - cs);
 #pragma unroll
-    for (int kk = 0; kk < C1; kk++) {
-      const int qq = l + kk * TPC;
-      const cpx<T> A = buf[pd(qq * RS + cs)];
-      if (cs == 0) {
-        // X[q][0] = Re Z + Im Z, X[q][M] = Re Z - Im Z, packed as X0 + i XM
-        x[kk] = {A.x + A.y, A.x - A.y};
-      } else {
-        x[kk] = s_post_r2c(A, buf[pd(qq * RS + M - cs)], twr[cs]);
+      for (int kk = 0; kk < C1; kk++) {
+        const cpx<T> A = cp[prow<RS>(kk * TPC)];
+        if (cs == 0) {
+          // X[q][0] = Re Z + Im Z, X[q][M] = Re Z - Im Z, packed as X0 + i XM
+          x[kk] = {A.x + A.y, A.x - A.y};
+        } else {
+          x[kk] = s_post_r2c(A, cm[prow<RS>(kk * TPC)], twr[cs]);
+        }
       }
     }
     bfly<C1, false>(x);
     __syncthreads();
-    // column exchange layout: element idx of column cs at pd(idx * M + cs)
+    // column exchange layout: element idx of column cs at row idx (stride M)
+    {
+      cpx<T>* cp = buf + prow<M>(l * C1) + pcol(cs);
 #pragma unroll
-    for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
+      for (int kk = 0; kk < C1; kk++) cp[prow<M>(kk)] = x[kk];
+    }
     __syncthreads();
     cpx<T> y[C1 / C2][C2];
+    {
+      const cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
 #pragma unroll
-    for (int b = 0; b < C1 / C2; b++) {
-      const int jj = l + b * TPC;
+      for (int b = 0; b < C1 / C2; b++) {
+        const int jj = l + b * TPC;
 #pragma unroll
-      for (int k = 0; k < C2; k++) {
-        y[b][k] = buf[pd((jj + k * C1) * M + cs)];
-        if (k) y[b][k] = cmul(y[b][k], twc[P * jj * k]);
+        for (int k = 0; k < C2; k++) {
+          y[b][k] = cp[prow<M>(b * TPC + k * C1)];
+          if (k) y[b][k] = cmul(y[b][k], twc[P * jj * k]);
+        }
+        bfly<C2, false>(y[b]);
       }
-      bfly<C2, false>(y[b]);
     }
     // packed column: Y0 = (Z[ka] + conj Z[-ka])/2, YM = -i (Z[ka] - conj Z[-ka])/2
     if (cs == 0) {
@@ -224,12 +252,14 @@ __global__ void __launch_bounds__(Q * M / R1)
   constexpr int TPR = M / R1;
   constexpr int TPC = Q / C1;
   constexpr int N1 = P * Q;
-  constexpr int RS = M + 1;  // rows keep X[q][M] for the C2R split
   static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
+  static_assert(32 % R1 == 0 && 32 % TPR == 0 && M % 32 == 0,
+                "LDS offsets split into lane + constant parts");
   extern __shared__ __align__(16) unsigned char smem[];
-  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
-  cpx<T>* twr = buf + pd_size(Q * RS);
+  cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);  // rows of M entries (padded)
+  cpx<T>* nyq = buf + pd_size(Q * M);              // X[q][M] of every row
+  cpx<T>* twr = nyq + Q;
   cpx<T>* twc = twr + 2 * M;
   const int tid = threadIdx.x;
 
@@ -256,19 +286,24 @@ __global__ void __launch_bounds__(Q * M / R1)
       x[kk] = val;
     }
     bfly<C1, true>(x);
+    {
+      // column exchange: element idx of column cs at row idx (stride M)
+      cpx<T>* cp = buf + prow<M>(l * C1) + pcol(cs);
 #pragma unroll
-    for (int kk = 0; kk < C1; kk++) buf[pd((l * C1 + kk) * M + cs)] = x[kk];
+      for (int kk = 0; kk < C1; kk++) cp[prow<M>(kk)] = x[kk];
+    }
     // prefetch the next work item
     if (PF && item + (int)gridDim.x < nitems)
       plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item + gridDim.x, tl);
     __syncthreads();
     cpx<T> y[C1 / C2][C2];
+    cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
 #pragma unroll
     for (int b = 0; b < C1 / C2; b++) {
       const int jj = l + b * TPC;
 #pragma unroll
       for (int k = 0; k < C2; k++) {
-        y[b][k] = buf[pd((jj + k * C1) * M + cs)];
+        y[b][k] = cp[prow<M>(b * TPC + k * C1)];
         if (k) y[b][k] = cmulc(y[b][k], twc[P * jj * k]);
       }
       bfly<C2, true>(y[b]);
@@ -279,39 +314,45 @@ __global__ void __launch_bounds__(Q * M / R1)
     for (int b = 0; b < C1 / C2; b++)
 #pragma unroll
       for (int k = 0; k < C2; k++) {
-        const int qq = l + b * TPC + k * C1;
         const cpx<T> val = y[b][k];
         if (cs == 0) {
-          buf[pd(qq * RS)] = {val.x, T(0)};
-          buf[pd(qq * RS + M)] = {val.y, T(0)};
+          cp[prow<M>(b * TPC + k * C1)] = {val.x, T(0)};
+          nyq[l + b * TPC + k * C1] = {val.y, T(0)};
         } else {
-          buf[pd(qq * RS + cs)] = val;
+          cp[prow<M>(b * TPC + k * C1)] = val;
         }
       }
     __syncthreads();
 
     // ---- rows: C2R split, then the inverse M-point transform
     cpx<T> v[R1];
+    {
+      const cpx<T>* rp = buf + prow<M>(q);
 #pragma unroll
-    for (int k = 0; k < R1; k++) {
-      const int mm = j + k * TPR;
-      const cpx<T> A = buf[pd(q * RS + mm)];
-      const cpx<T> B = buf[pd(q * RS + M - mm)];
-      // Z[0] = (X0 + conj XM) + i (X0 - conj XM) with X0, XM real
-      v[k] = mm == 0 ? cpx<T>{A.x + B.x, A.x - B.x} : s_pre_c2r(A, B, twr[mm]);
+      for (int k = 0; k < R1; k++) {
+        const int mm = j + k * TPR;
+        const cpx<T> A = rp[j + pcol(k * TPR)];
+        const cpx<T> B = mm == 0 ? nyq[q] : rp[pcol(M - mm)];
+        // Z[0] = (X0 + conj XM) + i (X0 - conj XM) with X0, XM real
+        v[k] = mm == 0 ? cpx<T>{A.x + B.x, A.x - B.x} : s_pre_c2r(A, B, twr[mm]);
+      }
     }
     bfly<R1, true>(v);
     __syncthreads();
+    {
+      cpx<T>* bp = buf + prow<M>(q) + pcol(j * R1);
 #pragma unroll
-    for (int k = 0; k < R1; k++) buf[pd(q * RS + j * R1 + k)] = v[k];
+      for (int k = 0; k < R1; k++) bp[k] = v[k];
+    }
     __syncthreads();
     cpx<T> w[R1 / R2][R2];
+    const cpx<T>* rp = buf + prow<M>(q) + j;
 #pragma unroll
     for (int b = 0; b < R1 / R2; b++) {
       const int jj = j + b * TPR;
 #pragma unroll
       for (int k = 0; k < R2; k++) {
-        w[b][k] = buf[pd(q * RS + jj + k * R1)];
+        w[b][k] = rp[b * TPR + pcol(k * R1)];
         if (k) w[b][k] = cmulc(w[b][k], twr[2 * jj * k]);
       }
       bfly<R2, true>(w[b]);
@@ -729,7 +770,7 @@ static hipError_t set_lds_s(K kernel, size_t lds) {
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
 struct PlaneS {
   static size_t lds_fwd() { return (pd_size(Q * M) + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
-  static size_t lds_inv() { return (pd_size(Q * (M + 1)) + 2 * M + P * Q) * sizeof(cpx<T>); }
+  static size_t lds_inv() { return (pd_size(Q * M) + Q + 2 * M + P * Q) * sizeof(cpx<T>); }
   static constexpr int threads() { return Q * M / R1; }
 };
 
