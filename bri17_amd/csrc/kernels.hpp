@@ -12,6 +12,9 @@
 //
 // Workspace layout (complex T): ws[c][i0][p][ka][k] with middle index
 // i1 = p*Q + ka (3D) and k in [0, rs), rs = NL/2+1; 2D: ws[c][i0][k].
+// The fixed-size kernels (kernels_static.hip) store k < NL/2 in rows of
+// NL/2 entries and the Nyquist entries k = NL/2 in a region of their own
+// after them (same total size).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -87,6 +90,9 @@ struct PencilParams {
   int f;        // flat extent of one p-block: q * rs
   int w;        // tile width (contiguous complex elements)
   int ntiles;
+  int nyq_tiles;    // fixed-size layout: the last nyq_tiles tiles walk the
+                    // Nyquist region (0: Nyquist entries in-row)
+  int64_t nyq_off;  // element offset of the Nyquist region
   int mode;     // 0: K u, 1: K^-1 f, 2: strain B u, 3: eigenstress -> -strain
   int xcd_remap;
   double mu, s;   // material constants with the apply / solve scale folded in

@@ -61,6 +61,10 @@ __device__ __forceinline__ void copy_table(cpx<T>* dst, const cpx<T>* src, int n
 // Row FFT M = R1 x R2 by M/R1 threads per row; column stage Q = C1 x C2 by
 // Q/C1 threads per column on M column sequences (columns 0 and M, both real
 // after the row transform, are packed into one complex column).
+//
+// Workspace: the spectrum rows (c, i0, p, ka) hold k < M, M entries each, so
+// every row starts on an M-entry boundary; the Nyquist entries k = M lie in a
+// region of their own, nyq[(c, i0, p, ka)], after the rows.
 
 template <typename T, int M, int R1, int Q, int P>
 __device__ __forceinline__ void plane_load_rows(cpx<T>* v, const T* __restrict__ u, int item,
@@ -88,8 +92,8 @@ __device__ __forceinline__ constexpr int pcol(int col) { return col + (col >> 5)
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
 __global__ void __launch_bounds__(Q * M / R1)
-    k_plane_fwd_s(const T* __restrict__ u, cpx<T>* __restrict__ ws, const cpx<T>* __restrict__ g_twr,
-                  const cpx<T>* __restrict__ g_twc, int rs, int nitems) {
+    k_plane_fwd_s(const T* __restrict__ u, cpx<T>* __restrict__ ws, cpx<T>* __restrict__ nyq,
+                  const cpx<T>* __restrict__ g_twr, const cpx<T>* __restrict__ g_twc, int nitems) {
   constexpr int NT = Q * M / R1;
   constexpr int TPR = M / R1;
   constexpr int TPC = Q / C1;
@@ -198,7 +202,8 @@ __global__ void __launch_bounds__(Q * M / R1)
         for (int k = 0; k < C2; k++) scr[l + b * TPC + k * C1] = y[b][k];
     }
     __syncthreads();
-    cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * rs;
+    cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * M;
+    cpx<T>* dn = nyq + ((int64_t)plane * N1 + p * Q);
 #pragma unroll
     for (int b = 0; b < C1 / C2; b++) {
 #pragma unroll
@@ -214,11 +219,11 @@ __global__ void __launch_bounds__(Q * M / R1)
             y0 = cmul(y0, tw);
             ym = cmul(ym, tw);
           }
-          dst[(int64_t)ka * rs] = y0;
-          dst[(int64_t)ka * rs + M] = ym;
+          dst[ka * M] = y0;
+          dn[ka] = ym;
         } else {
           if (P > 1 && p) val = cmul(val, twc[p * ka]);
-          dst[(int64_t)ka * rs + cs] = val;
+          dst[ka * M + cs] = val;
         }
       }
     }
@@ -230,24 +235,25 @@ __global__ void __launch_bounds__(Q * M / R1)
 
 template <typename T, int M, int C1, int Q, int P>
 __device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>* xm, const cpx<T>* __restrict__ ws,
-                                                int rs, int item, int tid) {
+                                                const cpx<T>* __restrict__ nyq, int item, int tid) {
   constexpr int TPC = Q / C1;
   const int plane = item / P, p = item - (item / P) * P;
   const int cs = tid % M, l = tid / M;
-  const cpx<T>* src = ws + ((int64_t)plane * (P * Q) + p * Q) * rs;
+  const cpx<T>* src = ws + ((int64_t)plane * (P * Q) + p * Q) * M;
+  const cpx<T>* sn = nyq + ((int64_t)plane * (P * Q) + p * Q);
 #pragma unroll
   for (int kk = 0; kk < C1; kk++) {
     const int ka = l + kk * TPC;
-    x[kk] = src[(int64_t)ka * rs + cs];
-    if (cs == 0) xm[kk] = src[(int64_t)ka * rs + M];
+    x[kk] = src[ka * M + cs];
+    if (cs == 0) xm[kk] = sn[ka];
   }
 }
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
 __global__ void __launch_bounds__(Q * M / R1)
-    k_plane_inv_s(const cpx<T>* __restrict__ ws, T* __restrict__ out,
-                  const cpx<T>* __restrict__ g_twr, const cpx<T>* __restrict__ g_twc, int rs,
-                  int nitems) {
+    k_plane_inv_s(const cpx<T>* __restrict__ ws, const cpx<T>* __restrict__ nyq,
+                  T* __restrict__ out, const cpx<T>* __restrict__ g_twr,
+                  const cpx<T>* __restrict__ g_twc, int nitems) {
   constexpr int NT = Q * M / R1;
   constexpr int TPR = M / R1;
   constexpr int TPC = Q / C1;
@@ -258,8 +264,8 @@ __global__ void __launch_bounds__(Q * M / R1)
                 "LDS offsets split into lane + constant parts");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);  // rows of M entries (padded)
-  cpx<T>* nyq = buf + pd_size(Q * M);              // X[q][M] of every row
-  cpx<T>* twr = nyq + Q;
+  cpx<T>* nyl = buf + pd_size(Q * M);              // X[q][M] of every row
+  cpx<T>* twr = nyl + Q;
   cpx<T>* twc = twr + 2 * M;
   const int tid = threadIdx.x;
 
@@ -269,10 +275,10 @@ __global__ void __launch_bounds__(Q * M / R1)
 
   cpx<T> x[C1], xm[C1];
   int item = blockIdx.x;
-  if (PF && item < nitems) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item, tid);
+  if (PF && item < nitems) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, nyq, item, tid);
   for (; item < nitems; item += gridDim.x) {
     const int tl = opaque(tid);
-    if (!PF) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item, tl);
+    if (!PF) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, nyq, item, tl);
     const int cs = tl % M, l = tl / M;
     const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
     const int plane = item / P;
@@ -294,7 +300,7 @@ __global__ void __launch_bounds__(Q * M / R1)
     }
     // prefetch the next work item
     if (PF && item + (int)gridDim.x < nitems)
-      plane_load_cols<T, M, C1, Q, P>(x, xm, ws, rs, item + gridDim.x, tl);
+      plane_load_cols<T, M, C1, Q, P>(x, xm, ws, nyq, item + gridDim.x, tl);
     __syncthreads();
     cpx<T> y[C1 / C2][C2];
     cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
@@ -317,7 +323,7 @@ __global__ void __launch_bounds__(Q * M / R1)
         const cpx<T> val = y[b][k];
         if (cs == 0) {
           cp[prow<M>(b * TPC + k * C1)] = {val.x, T(0)};
-          nyq[l + b * TPC + k * C1] = {val.y, T(0)};
+          nyl[l + b * TPC + k * C1] = {val.y, T(0)};
         } else {
           cp[prow<M>(b * TPC + k * C1)] = val;
         }
@@ -332,7 +338,7 @@ __global__ void __launch_bounds__(Q * M / R1)
       for (int k = 0; k < R1; k++) {
         const int mm = j + k * TPR;
         const cpx<T> A = rp[j + pcol(k * TPR)];
-        const cpx<T> B = mm == 0 ? nyq[q] : rp[pcol(M - mm)];
+        const cpx<T> B = mm == 0 ? nyl[q] : rp[pcol(M - mm)];
         // Z[0] = (X0 + conj XM) + i (X0 - conj XM) with X0, XM real
         v[k] = mm == 0 ? cpx<T>{A.x + B.x, A.x - B.x} : s_pre_c2r(A, B, twr[mm]);
       }
@@ -383,6 +389,32 @@ __device__ __forceinline__ void pencil_load(cpx<T> (&v)[P][A0], const cpx<T>* __
     for (int pp_ = 0; pp_ < P; pp_++) v[pp_][k] = src[((int64_t)(l + k * TPA) * P + pp_) * F];
 }
 
+// Pencil tiles of the fixed-size layout: tiles [0, ntiles - nyq_tiles) walk
+// the spectrum rows (F = Q M columns per row group (c, i0, p)), the last
+// nyq_tiles tiles the Nyquist region (F = Q columns, one per ka).
+struct STile {
+  int64_t off;  // element offset of the region
+  int F;        // columns per row group
+  int f0;       // first column of the tile
+  bool nyq;
+};
+__device__ __forceinline__ STile s_tile(const PencilParams& pp, int t, int W) {
+  const int ta = pp.ntiles - pp.nyq_tiles;
+  if (t < ta) return {0, pp.f, t * W, false};
+  return {pp.nyq_off, pp.q, (t - ta) * W, true};
+}
+// (ka, k) of column f of a tile: k = f mod M in the rows, k = M (= pp.rs) in
+// the Nyquist region
+__device__ __forceinline__ void s_freq(const PencilParams& pp, bool nyq, int f, int& ka, int& kl) {
+  if (nyq) {
+    ka = f;
+    kl = pp.rs;
+  } else {
+    ka = (int)fdiv((uint32_t)f, pp.rsdiv);
+    kl = f - ka * pp.rs;
+  }
+}
+
 // Persistent variant (PF): one workgroup per CU slot walks its tiles, and the
 // loads of its next tile are issued into the registers freed by the first
 // axis-0 pass, so they are in flight while the rest of the tile is computed.
@@ -402,7 +434,6 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
   const ModalCoef<T>* coef1 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]);
   const ModalCoef<T>* coef2 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[2]);
   const int tid = threadIdx.x;
-  const int F = pp.f;
   const int T_ = pp.ntiles;
 
   // tiles of this workgroup: first + j * stride < cnt, offset by base
@@ -423,19 +454,21 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
 
   copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, NT);
 
-  auto column = [&](int tile, int& f0, int& wv, int& fw) {
-    f0 = tile * W;
-    wv = F - f0 < W ? F - f0 : W;
+  // tile -> region and columns; fw: this lane's column
+  auto column = [&](int tile, STile& st, int& wv, int& fw) {
+    st = s_tile(pp, tile, W);
+    wv = st.F - st.f0 < W ? st.F - st.f0 : W;
     const int w = tid % W;
     // out-of-range columns of the last tile read a valid address and are never stored
-    fw = w < wv ? f0 + w : F - 1;
+    fw = w < wv ? st.f0 + w : st.F - 1;
   };
 
   cpx<T> v[P][A0];
   {
-    int f0, wv, fw;
-    column(base + first, f0, wv, fw);
-    pencil_load<T, N0, A0, P, W>(v, ws, F, tid / (W * TPA), (tid / W) % TPA, fw);
+    STile st;
+    int wv, fw;
+    column(base + first, st, wv, fw);
+    pencil_load<T, N0, A0, P, W>(v, ws + st.off, st.F, tid / (W * TPA), (tid / W) % TPA, fw);
   }
   for (int j = first; j < cnt; j += stride) {
     const int tl = opaque(tid);
@@ -443,8 +476,10 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
     const int w = tl % W;
     const int l = (tl / W) % TPA;
     const int c = tl / (W * TPA);
-    int f0, wv, fw;
-    column(t, f0, wv, fw);
+    STile st;
+    int wv, fw;
+    column(t, st, wv, fw);
+    const int F = st.F, f0 = st.f0;
     const bool wok = w < wv;
     if (PF && j > first) __syncthreads();  // the previous tile's last LDS reads are done
 
@@ -468,9 +503,10 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
     }
     // prefetch the next tile into the now free registers
     if (PF && j + stride < cnt) {
-      int nf0, nwv, nfw;
-      column(t + stride, nf0, nwv, nfw);
-      pencil_load<T, N0, A0, P, W>(v, ws, F, c, l, nfw);
+      STile nst;
+      int nwv, nfw;
+      column(t + stride, nst, nwv, nfw);
+      pencil_load<T, N0, A0, P, W>(v, ws + nst.off, nst.F, c, l, nfw);
     }
     __syncthreads();
     // pass 2 -> natural order k0 = jj + A0 k
@@ -506,10 +542,9 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
       const int bw = tl % W;
       const int bk = (tl / W) % KT;
       const int bkb = tl / (W * KT);
-      const int f = f0 + bw;
-      const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
-      const int kl = f - ka * pp.rs;
-      if (bw < wv && kl < pp.nlh) {
+      int ka, kl;
+      s_freq(pp, st.nyq, f0 + bw, ka, kl);
+      if (bw < wv) {
         const int k1 = ka + pp.q * bkb;
         const T mu = T(pp.mu), s = T(pp.s);
         const ModalCoef<T> b = coef1[k1], cc = coef2[kl];
@@ -578,7 +613,7 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
           }
       }
       if (wok) {
-        cpx<T>* dst = ws + (int64_t)c * N0 * P * F + f0 + w;
+        cpx<T>* dst = ws + st.off + (int64_t)c * N0 * P * F + f0 + w;
 #pragma unroll
         for (int b = 0; b < A0 / B0; b++)
 #pragma unroll
@@ -664,7 +699,7 @@ __device__ __forceinline__ int split_bitrev(int p) {
 
 template <typename T, int R, int P, int W, int MODE>
 __global__ void __launch_bounds__(W * R * P)
-    k_pencil_r(cpx<T>* __restrict__ ws, const PencilParams pp, int ntiles) {
+    k_pencil_r(cpx<T>* __restrict__ ws, const PencilParams pp) {
   constexpr int N0 = R * R;
   static_assert(P * W <= 64, "the split group must lie in one wavefront");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -681,9 +716,11 @@ __global__ void __launch_bounds__(W * R * P)
     const int xx = t % 8, idx = t / 8;
     t = xx * per + (xx < rem ? xx : rem) + idx;
   }
-  const int F = pp.f;
-  const int f0 = t * W;
+  const STile st = s_tile(pp, t, W);
+  const int F = st.F;
+  const int f0 = st.f0;
   const int wv = F - f0 < W ? F - f0 : W;
+  cpx<T>* const wsr = ws + st.off;
   const int w = tid % W;
   const int p = (tid / W) % P;
   const int l = tid / (W * P);
@@ -699,7 +736,7 @@ __global__ void __launch_bounds__(W * R * P)
     const int lw = tl % W;
     // out-of-range columns of the last tile read a valid address and are never stored
     const int lf = lw < wv ? f0 + lw : F - 1;
-    return ws + (int64_t)(ll * P + lp) * F + lf + (int64_t)c * N0 * P * F;
+    return wsr + (int64_t)(ll * P + lp) * F + lf + (int64_t)c * N0 * P * F;
   };
 
   cpx<T> X[3][R];
@@ -715,10 +752,9 @@ __global__ void __launch_bounds__(W * R * P)
 
   // modal block at k0 = l + R j, k1 = ka + Q kb
   {
-    const int f = f0 + w;
-    const int ka = (int)fdiv((uint32_t)f, pp.rsdiv);
-    const int kl = f - ka * pp.rs;
-    if (wok && kl < pp.nlh) {
+    int ka, kl;
+    s_freq(pp, st.nyq, f0 + w, ka, kl);
+    if (wok) {
       const T mu = T(pp.mu), s = T(pp.s);
       const ModalCoef<T> cc = coef2[kl];
       const int k1 = ka + pp.q * kb;
@@ -780,11 +816,17 @@ struct PencilS {
   static constexpr int threads() { return 3 * W * (N0 / A0); }
 };
 
+// Tiles of width W over the fixed-size layout (rows, then Nyquist region).
+static void set_tiles(PencilParams& pc, int W) {
+  pc.nyq_tiles = (pc.q + W - 1) / W;
+  pc.ntiles = (pc.f + W - 1) / W + pc.nyq_tiles;
+}
+
 // Pencil launch of a fixed-size case: the register-block pencil for N0 = R^2
 // (fp32, P = 2 or 4) unless bit 3 of `pf` asks for the all-components-in-LDS
 // pencil; bit 2: persistent variant of the latter.
 template <typename T, int N0, int A0, int B0, int P, int W>
-static hipError_t launch_pencil_static(cpx<T>* ws, const PencilParams& pc, int mode, int pf,
+static hipError_t launch_pencil_static(cpx<T>* ws, PencilParams pc, int mode, int pf,
                                        hipStream_t st) {
   if constexpr (A0 == B0 && A0 * B0 == N0 && (P == 2 || P == 4) && sizeof(T) == 4) {
     if (!(pf & 8)) {
@@ -793,11 +835,12 @@ static hipError_t launch_pencil_static(cpx<T>* ws, const PencilParams& pc, int m
       const size_t lds = (pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>);
       hipError_t e = set_lds_s(kp, lds);
       if (e != hipSuccess) return e;
-      const int nt = (pc.f + WR - 1) / WR;
-      hipLaunchKernelGGL(kp, dim3(nt), dim3(WR * A0 * P), lds, st, ws, pc, nt);
+      set_tiles(pc, WR);
+      hipLaunchKernelGGL(kp, dim3(pc.ntiles), dim3(WR * A0 * P), lds, st, ws, pc);
       return hipGetLastError();
     }
   }
+  set_tiles(pc, W);
   using KS = PencilS<T, N0, A0, B0, P, W>;
   auto kp = mode == 0 ? ((pf & 4) ? k_pencil_s<T, N0, A0, B0, P, W, 0, true>
                                   : k_pencil_s<T, N0, A0, B0, P, W, 0, false>)
@@ -877,8 +920,8 @@ std::string static_describe(int dtype, int n0, int n1, int n2, int pf) {
 }
 
 hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void* in, void* out,
-                         void* ws, const PencilParams& pc, const void* twr, const void* twc,
-                         int rs, int pf, hipStream_t st, hipEvent_t* ev) {
+                         void* ws, const PencilParams& pc_in, const void* twr, const void* twc,
+                         int pf, hipStream_t st, hipEvent_t* ev) {
   hipError_t e = hipErrorInvalidValue;
   auto mark = [&](int i) {
     if (ev && e == hipSuccess) e = hipEventRecord(ev[i], st);
@@ -893,12 +936,19 @@ hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void
     e = set_lds_s(kf, PS::lds_fwd());                                                             \
     if (e == hipSuccess) e = set_lds_s(ki, PS::lds_inv());                                        \
     const int nitems = 3 * A * P;                                                                 \
+    /* rows of M entries, then the Nyquist region */                                              \
+    PencilParams pc = pc_in;                                                                      \
+    pc.rs = M;                                                                                    \
+    pc.rsdiv = make_fastdiv(M);                                                                   \
+    pc.f = Q * M;                                                                                 \
+    pc.nyq_off = (int64_t)3 * A * B * M;                                                          \
+    cpx<T>* nyq = (cpx<T>*)ws + pc.nyq_off;                                                       \
     const int gf = (pf & 1) ? resident_blocks(kf, PS::threads(), PS::lds_fwd(), nitems) : nitems; \
     const int gi = (pf & 2) ? resident_blocks(ki, PS::threads(), PS::lds_inv(), nitems) : nitems; \
     mark(0);                                                                                      \
     if (e == hipSuccess) {                                                                        \
       hipLaunchKernelGGL(kf, dim3(gf), dim3(PS::threads()), PS::lds_fwd(), st, (const T*)in,      \
-                         (cpx<T>*)ws, (const cpx<T>*)twr, (const cpx<T>*)twc, rs, nitems);        \
+                         (cpx<T>*)ws, nyq, (const cpx<T>*)twr, (const cpx<T>*)twc, nitems);       \
       e = hipGetLastError();                                                                      \
     }                                                                                             \
     mark(1);                                                                                      \
@@ -908,8 +958,8 @@ hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void
     mark(2);                                                                                      \
     if (e == hipSuccess) {                                                                        \
       hipLaunchKernelGGL(ki, dim3(gi), dim3(PS::threads()), PS::lds_inv(), st,                    \
-                         (const cpx<T>*)ws, (T*)out, (const cpx<T>*)twr, (const cpx<T>*)twc, rs,  \
-                         nitems);                                                                 \
+                         (const cpx<T>*)ws, (const cpx<T>*)nyq, (T*)out, (const cpx<T>*)twr,      \
+                         (const cpx<T>*)twc, nitems);                                             \
       e = hipGetLastError();                                                                      \
     }                                                                                             \
     mark(3);                                                                                      \

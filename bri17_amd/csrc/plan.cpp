@@ -163,7 +163,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
                           tun.radices[1].empty() && tun.radices[2].empty() &&
                           static_supported(dtype, shape[0], shape[1], shape[2], &sg);
   if (use_static) {
-    // the generic description shares the workspace layout of the fixed-size path
+    // the generic description has the split and tile width of the fixed-size path
     tun.plane_split = sg.p;
     tun.pencil_width = sg.w;
   }
@@ -505,8 +505,7 @@ Status run_pipeline(const Plan& pl, double mu, double nu, int mode, const void* 
   hipError_t e = hipSuccess;
   if (pl.use_static) {
     e = launch_static(pl.dtype, pl.shape[0], pl.shape[1], pl.shape[2], mode, in, out, ws, pc,
-                      pl.plane.tw_row, pl.plane.tw_col, pl.plane.rs, pl.tuning.plane_prefetch, st,
-                      ev);
+                      pl.plane.tw_row, pl.plane.tw_col, pl.tuning.plane_prefetch, st, ev);
     if (e != hipSuccess) return fail(3, std::string("kernel launch: ") + hipGetErrorString(e));
     return Status{};
   }
