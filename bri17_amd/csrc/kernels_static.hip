@@ -49,6 +49,55 @@ __device__ __forceinline__ int opaque(int v) {
   return v;
 }
 
+// One complex entry stored through a buffer resource.  A lane whose byte
+// offset lies beyond the resource stores nothing, so a per-lane condition
+// needs no branch around the store: with every global access of a persistent
+// loop outside divergent branches, the compiler's count of memory operations
+// in flight at the loop head stays exact, and the wait for the prefetched
+// loads does not also wait for the previous work item's stores.
+constexpr uint32_t kBufferNone = 0x80000000u;  // beyond every resource used here
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(cpx<T>* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ void buffer_store(cpx<T> v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 d = {__float_as_uint(v.x), __float_as_uint(v.y)};
+    __builtin_amdgcn_raw_buffer_store_b64(d, r, (int)off, 0, 0);
+  } else {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const unsigned long long a = __double_as_longlong(v.x), b = __double_as_longlong(v.y);
+    const u4 d = {(unsigned)a, (unsigned)(a >> 32), (unsigned)b, (unsigned)(b >> 32)};
+    __builtin_amdgcn_raw_buffer_store_b128(d, r, (int)off, 0, 0);
+  }
+}
+
+// s_waitcnt immediate that waits until at most n vector memory operations of
+// the wave are in flight (gfx9 encoding: vmcnt in bits 3:0 and 15:14; expcnt
+// and lgkmcnt left at their no-wait maxima).  The compiler's wait insertion
+// takes such an explicit wait into account.
+__host__ __device__ constexpr int vmcnt_wait(int n) {
+  return ((n & 15) | ((n >> 4) << 14)) | (7 << 4) | (15 << 8);
+}
+
+// One complex entry loaded through a buffer resource (0 for a lane whose
+// offset lies beyond the resource: no memory access, no branch).
+template <typename T>
+__device__ __forceinline__ cpx<T> buffer_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  if constexpr (sizeof(T) == 4) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 d = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    return {__uint_as_float(d.x), __uint_as_float(d.y)};
+  } else {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const u4 d = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+    return {__longlong_as_double((long long)(((unsigned long long)d.y << 32) | d.x)),
+            __longlong_as_double((long long)(((unsigned long long)d.w << 32) | d.z))};
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void copy_table(cpx<T>* dst, const cpx<T>* src, int n, int tid,
                                            int nthr) {
@@ -102,20 +151,26 @@ __global__ void __launch_bounds__(Q * M / R1)
   static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
   static_assert(32 % R1 == 0 && M % 32 == 0, "LDS offsets split into lane + constant parts");
+  static_assert((M & (M - 1)) == 0 && (Q & (Q - 1)) == 0, "power-of-two plane sides");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
   cpx<T>* twr = buf + pd_size(Q * RS);   // w_{2M}^j, j < 2M
   cpx<T>* twc = twr + 2 * M;             // w_{N1}^j, j < N1
   cpx<T>* scr = twc + N1;                // packed-column scratch, Q entries
+  cpx<T>* scn = scr + Q;                 // Nyquist entries of the item, Q entries
   const int tid = threadIdx.x;
+  const __amdgpu_buffer_rsrc_t rn = buffer_rsrc(nyq, (uint32_t)(nitems * Q * sizeof(cpx<T>)));
 
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
 
   cpx<T> v[R1];
   int item = blockIdx.x;
-  if (PF && item < nitems) plane_load_rows<T, M, R1, Q, P>(v, u, item, tid);
-  for (; item < nitems; item += gridDim.x) {
+  if (PF && item < nitems) {
+    plane_load_rows<T, M, R1, Q, P>(v, u, item, tid);
+    __builtin_amdgcn_s_waitcnt(vmcnt_wait(0));
+  }
+  auto process = [&](const int item) {
     const int tl = opaque(tid);
     if (!PF) plane_load_rows<T, M, R1, Q, P>(v, u, item, tl);
     const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
@@ -155,20 +210,24 @@ __global__ void __launch_bounds__(Q * M / R1)
     __syncthreads();
 
     // ---- columns: pass 1 reads Z of two columns and forms the spectrum;
-    // rows qq = l + kk TPC of columns cs and M - cs
+    // rows qq = l + kk TPC of columns cs and M - cs.  Lane cs = 0 reads column
+    // 0 twice: the same formula gives X[q][0] = Re Z + Im Z (real), and the
+    // lane packs X0 + i XM with X[q][M] = Re Z - Im Z (no divergent path).
     cpx<T> x[C1];
     {
       const cpx<T>* cp = buf + prow<RS>(l) + pcol(cs);
-      const cpx<T>* cm = buf + prow<RS>(l) + pcol(M - cs);
+      const cpx<T>* cm = buf + prow<RS>(l) + pcol((M - cs) & (M - 1));
+      const cpx<T> wc = twr[cs];
+      cpx<T> A[C1], B[C1];
 #pragma unroll
       for (int kk = 0; kk < C1; kk++) {
-        const cpx<T> A = cp[prow<RS>(kk * TPC)];
-        if (cs == 0) {
-          // X[q][0] = Re Z + Im Z, X[q][M] = Re Z - Im Z, packed as X0 + i XM
-          x[kk] = {A.x + A.y, A.x - A.y};
-        } else {
-          x[kk] = s_post_r2c(A, cm[prow<RS>(kk * TPC)], twr[cs]);
-        }
+        A[kk] = cp[prow<RS>(kk * TPC)];
+        B[kk] = cm[prow<RS>(kk * TPC)];
+      }
+#pragma unroll
+      for (int kk = 0; kk < C1; kk++) {
+        x[kk] = s_post_r2c(A[kk], B[kk], wc);
+        x[kk].y = cs == 0 ? A[kk].x - A[kk].y : x[kk].y;
       }
     }
     bfly<C1, false>(x);
@@ -202,51 +261,61 @@ __global__ void __launch_bounds__(Q * M / R1)
         for (int k = 0; k < C2; k++) scr[l + b * TPC + k * C1] = y[b][k];
     }
     __syncthreads();
-    cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * M;
-    cpx<T>* dn = nyq + ((int64_t)plane * N1 + p * Q);
+    // lanes cs = 0 unpack: Y0 replaces the packed value, YM (twiddled) is
+    // staged in LDS for the Nyquist region
+    if (cs == 0) {
 #pragma unroll
-    for (int b = 0; b < C1 / C2; b++) {
+      for (int b = 0; b < C1 / C2; b++)
 #pragma unroll
-      for (int k = 0; k < C2; k++) {
-        const int ka = l + b * TPC + k * C1;
-        cpx<T> val = y[b][k];
-        if (cs == 0) {
+        for (int k = 0; k < C2; k++) {
+          const int ka = l + b * TPC + k * C1;
+          const cpx<T> val = y[b][k];
           const cpx<T> zn = scr[(Q - ka) % Q];
-          cpx<T> y0 = {T(0.5) * (val.x + zn.x), T(0.5) * (val.y - zn.y)};
+          y[b][k] = {T(0.5) * (val.x + zn.x), T(0.5) * (val.y - zn.y)};
           cpx<T> ym = {T(0.5) * (val.y + zn.y), T(0.5) * (zn.x - val.x)};
-          if (P > 1 && p) {
-            const cpx<T> tw = twc[p * ka];
-            y0 = cmul(y0, tw);
-            ym = cmul(ym, tw);
-          }
-          dst[ka * M] = y0;
-          dn[ka] = ym;
-        } else {
-          if (P > 1 && p) val = cmul(val, twc[p * ka]);
-          dst[ka * M + cs] = val;
+          if (P > 1 && p) ym = cmul(ym, twc[p * ka]);
+          scn[ka] = ym;
         }
-      }
     }
+    if (P > 1 && p) {
+#pragma unroll
+      for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+        for (int k = 0; k < C2; k++) y[b][k] = cmul(y[b][k], twc[p * (l + b * TPC + k * C1)]);
+    }
+    cpx<T>* dst = ws + ((int64_t)plane * N1 + p * Q) * M + cs;
+#pragma unroll
+    for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+      for (int k = 0; k < C2; k++) dst[(l + b * TPC + k * C1) * M] = y[b][k];
     __syncthreads();  // LDS is rewritten by the next work item
-  }
+    // the item's Nyquist entries: thread tid < Q stores entry tid (scn is
+    // rewritten only after several barriers of the next item)
+    buffer_store(scn[tid & (Q - 1)], rn,
+                 tid < Q ? (uint32_t)(((plane * N1 + p * Q) + tid) * sizeof(cpx<T>)) : kBufferNone);
+    // The next item's loads were issued before this item's C1 + 1 stores:
+    // waiting for them here leaves the stores in flight across the loop head.
+    static_assert(C1 + 1 < 64, "vmcnt range");
+    if (PF) __builtin_amdgcn_s_waitcnt(vmcnt_wait(C1 + 1));
+  };
+  for (; item < nitems; item += gridDim.x) process(item);
 }
 
 // ------------------------------------------------------------------ plane inv
 
+// Loads of work item (c, i0, p): column entries (ka = l + kk TPC, cs) into x,
+// and Nyquist entry ka = tid into xn for threads tid < Q (no branch).
 template <typename T, int M, int C1, int Q, int P>
-__device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>* xm, const cpx<T>* __restrict__ ws,
-                                                const cpx<T>* __restrict__ nyq, int item, int tid) {
+__device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>& xn, const cpx<T>* __restrict__ ws,
+                                                __amdgpu_buffer_rsrc_t rn, int item, int tid) {
   constexpr int TPC = Q / C1;
   const int plane = item / P, p = item - (item / P) * P;
   const int cs = tid % M, l = tid / M;
   const cpx<T>* src = ws + ((int64_t)plane * (P * Q) + p * Q) * M;
-  const cpx<T>* sn = nyq + ((int64_t)plane * (P * Q) + p * Q);
 #pragma unroll
-  for (int kk = 0; kk < C1; kk++) {
-    const int ka = l + kk * TPC;
-    x[kk] = src[ka * M + cs];
-    if (cs == 0) xm[kk] = sn[ka];
-  }
+  for (int kk = 0; kk < C1; kk++) x[kk] = src[(l + kk * TPC) * M + cs];
+  xn = buffer_load<T>(rn, tid < Q ? (uint32_t)((plane * (P * Q) + p * Q + tid) * sizeof(cpx<T>))
+                                  : kBufferNone);
 }
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
@@ -267,18 +336,29 @@ __global__ void __launch_bounds__(Q * M / R1)
   cpx<T>* nyl = buf + pd_size(Q * M);              // X[q][M] of every row
   cpx<T>* twr = nyl + Q;
   cpx<T>* twc = twr + 2 * M;
+  cpx<T>* nin = twc + N1;                          // the item's Nyquist entries
   const int tid = threadIdx.x;
+  const __amdgpu_buffer_rsrc_t rn =
+      buffer_rsrc(const_cast<cpx<T>*>(nyq), (uint32_t)(nitems * Q * sizeof(cpx<T>)));
 
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
-  __syncthreads();
 
-  cpx<T> x[C1], xm[C1];
+  cpx<T> x[C1], xn;
   int item = blockIdx.x;
-  if (PF && item < nitems) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, nyq, item, tid);
+  if (PF && item < nitems) {
+    plane_load_cols<T, M, C1, Q, P>(x, xn, ws, rn, item, tid);
+    __builtin_amdgcn_s_waitcnt(vmcnt_wait(0));
+    if (tid < Q) nin[tid] = xn;
+  }
+  __syncthreads();
   for (; item < nitems; item += gridDim.x) {
     const int tl = opaque(tid);
-    if (!PF) plane_load_cols<T, M, C1, Q, P>(x, xm, ws, nyq, item, tl);
+    if (!PF) {
+      plane_load_cols<T, M, C1, Q, P>(x, xn, ws, rn, item, tl);
+      if (tl < Q) nin[tl] = xn;
+      __syncthreads();
+    }
     const int cs = tl % M, l = tl / M;
     const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
     const int plane = item / P;
@@ -287,7 +367,8 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
     for (int kk = 0; kk < C1; kk++) {
       cpx<T> val = x[kk];
-      if (cs == 0) val = {val.x - xm[kk].y, val.y + xm[kk].x};  // X0 + i XM (both Hermitian)
+      const cpx<T> xm = nin[l + kk * TPC];  // same address across the wave
+      val = cs == 0 ? cpx<T>{val.x - xm.y, val.y + xm.x} : val;  // X0 + i XM (both Hermitian)
       if (P > 1 && p) val = cmulc(val, twc[p * (l + kk * TPC)]);
       x[kk] = val;
     }
@@ -300,7 +381,7 @@ __global__ void __launch_bounds__(Q * M / R1)
     }
     // prefetch the next work item
     if (PF && item + (int)gridDim.x < nitems)
-      plane_load_cols<T, M, C1, Q, P>(x, xm, ws, nyq, item + gridDim.x, tl);
+      plane_load_cols<T, M, C1, Q, P>(x, xn, ws, rn, item + gridDim.x, tl);
     __syncthreads();
     cpx<T> y[C1 / C2][C2];
     cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
@@ -321,26 +402,30 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
       for (int k = 0; k < C2; k++) {
         const cpx<T> val = y[b][k];
-        if (cs == 0) {
-          cp[prow<M>(b * TPC + k * C1)] = {val.x, T(0)};
-          nyl[l + b * TPC + k * C1] = {val.y, T(0)};
-        } else {
-          cp[prow<M>(b * TPC + k * C1)] = val;
-        }
+        cp[prow<M>(b * TPC + k * C1)] = cs == 0 ? cpx<T>{val.x, T(0)} : val;
       }
+    if (cs == 0) {
+#pragma unroll
+      for (int b = 0; b < C1 / C2; b++)
+#pragma unroll
+        for (int k = 0; k < C2; k++) nyl[l + b * TPC + k * C1] = {y[b][k].y, T(0)};
+    }
     __syncthreads();
 
     // ---- rows: C2R split, then the inverse M-point transform
     cpx<T> v[R1];
     {
       const cpx<T>* rp = buf + prow<M>(q);
+      // entry mm = j + k TPR pairs with M - mm; mm = 0 (lane j = 0, k = 0)
+      // pairs with X[q][M], and the same formula with w = 1 gives
+      // Z[0] = (X0 + XM) + i (X0 - XM) for X0, XM real
 #pragma unroll
       for (int k = 0; k < R1; k++) {
         const int mm = j + k * TPR;
         const cpx<T> A = rp[j + pcol(k * TPR)];
-        const cpx<T> B = mm == 0 ? nyl[q] : rp[pcol(M - mm)];
-        // Z[0] = (X0 + conj XM) + i (X0 - conj XM) with X0, XM real
-        v[k] = mm == 0 ? cpx<T>{A.x + B.x, A.x - B.x} : s_pre_c2r(A, B, twr[mm]);
+        const cpx<T>* bp = rp + pcol(M - mm);
+        if (k == 0) bp = j == 0 ? nyl + q : bp;
+        v[k] = s_pre_c2r(A, *bp, twr[mm]);
       }
     }
     bfly<R1, true>(v);
@@ -368,6 +453,13 @@ __global__ void __launch_bounds__(Q * M / R1)
     for (int b = 0; b < R1 / R2; b++)
 #pragma unroll
       for (int k = 0; k < R2; k++) dst[j + b * TPR + k * R1] = w[b][k];
+    if (PF) {
+      // The next item's loads were issued before this item's R1 stores:
+      // waiting for them leaves the stores in flight across the loop head.
+      static_assert(R1 < 64, "vmcnt range");
+      __builtin_amdgcn_s_waitcnt(vmcnt_wait(R1));
+      if (tl < Q) nin[tl] = xn;
+    }
     __syncthreads();  // LDS is rewritten by the next work item
   }
 }
@@ -675,7 +767,18 @@ __device__ __forceinline__ cpx<T> split_dft(cpx<T> x, int p) {
     const cpx<T> o = shfl_xor_c(v, m);
     return second ? cpx<T>{o.x - v.x, o.y - v.y} : cpx<T>{v.x + o.x, v.y + o.y};
   };
-  if constexpr (P == 2) {
+  if constexpr (P == 2 && W == 16 && sizeof(T) == 4) {
+    // partner lane = lane ^ 16: v_permlane16_swap of x with itself hands
+    // every lane of a 16-lane row pair the even row's value (first result)
+    // and the odd row's (second result); lane p keeps x_0 + (-1)^p x_1
+    const auto sx = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.x), __float_as_uint(x.x),
+                                                     false, false);
+    const auto sy = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.y), __float_as_uint(x.y),
+                                                     false, false);
+    const T sg = p & 1 ? T(-1) : T(1);
+    return {__builtin_fmaf(sg, __uint_as_float(sx[1]), __uint_as_float(sx[0])),
+            __builtin_fmaf(sg, __uint_as_float(sy[1]), __uint_as_float(sy[0]))};
+  } else if constexpr (P == 2) {
     return bfly_lanes(x, W, p & 1);
   } else {
     static_assert(P == 4, "split of 2 or 4");
@@ -805,8 +908,8 @@ static hipError_t set_lds_s(K kernel, size_t lds) {
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
 struct PlaneS {
-  static size_t lds_fwd() { return (pd_size(Q * M) + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
-  static size_t lds_inv() { return (pd_size(Q * M) + Q + 2 * M + P * Q) * sizeof(cpx<T>); }
+  static size_t lds_fwd() { return (pd_size(Q * M) + 2 * M + P * Q + 2 * Q) * sizeof(cpx<T>); }
+  static size_t lds_inv() { return (pd_size(Q * M) + Q + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
   static constexpr int threads() { return Q * M / R1; }
 };
 

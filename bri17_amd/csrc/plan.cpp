@@ -221,7 +221,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   pc.n0 = shape[0];
   pc.rs = nlh;
   pc.nlh = nlh;
-  pc.xcd_remap = tun.xcd_remap;
+  pc.xcd_remap = tun.xcd_remap >= 0 ? tun.xcd_remap : (use_static ? 0 : 1);
 
   if (dim == 3) {
     const int n1 = shape[1];

@@ -27,9 +27,9 @@ struct Tuning {
   int plane_split = 0;       // middle-axis split P (3D); 0 = automatic
   int pencil_width = 0;      // pencil tile width W; 0 = automatic
   size_t lds_budget = 160 * 1024;
-  int xcd_remap = 1;
+  int xcd_remap = -1;           // -1: off for the fixed-size kernels, on otherwise
   int force_generic = 0;        // 1: never use the fixed-size kernels
-  int plane_prefetch = 1;       // fixed-size kernels persistent: bit0 fwd, bit1 inv, bit2 pencil
+  int plane_prefetch = 3;       // fixed-size kernels persistent: bit0 fwd, bit1 inv, bit2 pencil
   std::vector<int> radices[3];  // forced radix orders per FFT length role
 };
 

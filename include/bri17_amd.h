@@ -97,7 +97,8 @@ typedef struct bri17_tuning {
                          prime factors <= 7) */
   int pencil_width;   /* contiguous spectrum entries per pencil tile */
   int lds_budget_kb;  /* LDS per workgroup, KiB (<= 160) */
-  int xcd_remap;      /* 0: off, 1: on, -1: automatic (on) */
+  int xcd_remap;      /* 0: off, 1: on, -1: automatic (off for the fixed-size
+                         kernels, on for the runtime-size ones) */
   int radices[3][8];  /* radix order of the row / middle-axis / axis-0
                          transforms, 0-terminated (radices 2,3,4,5,7,8,9,16) */
   int force_generic;  /* 1: use the runtime-size kernels even where a

@@ -192,6 +192,33 @@ def test_fixed_size_plan_description(device):
     assert "fixed_size" not in StiffnessMatrix((48, 36, 54), None, 1.0, 0.3, "float32").describe()
 
 
+@pytest.mark.parametrize("shape,dtype", [((256, 256, 256), "float32"), ((128, 128, 128), "float32"),
+                                         ((64, 64, 64), "float64")])
+def test_fixed_size_launch_variants(device, shape, dtype):
+    """Every launch variant of the fixed-size kernels gives the default
+    variant's result: the persistent prefetching plane kernels (bits 0 and 1
+    of plane_prefetch) and the persistent LDS pencil (bit 2) bitwise, the LDS
+    pencil in place of the register-block one (bit 3, 256^3) within 1e-6.  The
+    default variant itself is checked against the oracle above."""
+    import torch
+    from bri17_amd import StiffnessMatrix
+    g = torch.Generator(device=device).manual_seed(7)
+    u = torch.rand((3,) + shape, generator=g, device=device, dtype=getattr(torch, dtype)) * 2 - 1
+    reg = StiffnessMatrix(shape, None, 1.1, 0.3, dtype).describe()["fixed_size"]["pencil"]
+    for mode in ("apply", "solve"):
+        ref = getattr(StiffnessMatrix(shape, None, 1.1, 0.3, dtype), mode)(u)
+        for pf in range(16):
+            op = StiffnessMatrix(shape, None, 1.1, 0.3, dtype, tuning=dict(plane_prefetch=pf))
+            got = getattr(op, mode)(u)
+            torch.cuda.synchronize()
+            print(shape, dtype, mode, "plane_prefetch", pf, flush=True)
+            if reg == "register-block" and pf & 8:
+                assert rel_err(got.cpu().numpy(), ref.cpu().numpy()) <= 1e-6, (mode, pf)
+            else:
+                assert torch.equal(got, ref), (mode, pf)
+            del op
+
+
 @pytest.mark.parametrize("split", [6, 9, 18, 27])
 @pytest.mark.parametrize("mode", ["apply", "solve"])
 def test_middle_axis_splits_without_fused_butterfly(device, oracle, split, mode):
