@@ -724,13 +724,12 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
 // For N0 = R x R the axis-0 transform is one radix-R pass in registers, an
 // LDS transposition, and a second radix-R pass, and the output of the second
 // pass holds k0 = l + R j: exactly the input set of the inverse's first pass.
-// The components are therefore transformed one after the other through a
-// one-component LDS buffer, the modal block is applied in registers (a thread
-// holds all DIM components of its frequencies), and each component is
-// transformed back and stored.  LDS per tile is one component (P N0 W
-// entries), so W = 16 tiles (128-byte rows) fit twice per CU.  The split
-// index p is a lane bit (partner lane = lane ^ W): the P = 2 split DFT is a
-// lane exchange.
+// The components are therefore transformed one after the other through
+// one-component LDS buffers (P N0 W entries; two of them alternate), the
+// modal block is applied in registers (a thread holds all DIM components of
+// its frequencies), and each component is transformed back and stored.  The
+// split index p is a lane bit (partner lane = lane ^ W): the P = 2 split DFT
+// is a lane exchange.
 
 template <typename T, int R, int W, bool INV>
 __device__ __forceinline__ void pencil_r_fft(cpx<T> (&v)[R], cpx<T>* buf, const cpx<T>* tw,
@@ -748,7 +747,6 @@ __device__ __forceinline__ void pencil_r_fft(cpx<T> (&v)[R], cpx<T>* buf, const 
     v[i] = x;
   }
   bfly<R, INV>(v);
-  __syncthreads();  // the buffer is reused by the next component
 }
 
 template <typename T>
@@ -768,16 +766,19 @@ __device__ __forceinline__ cpx<T> split_dft(cpx<T> x, int p) {
     return second ? cpx<T>{o.x - v.x, o.y - v.y} : cpx<T>{v.x + o.x, v.y + o.y};
   };
   if constexpr (P == 2 && W == 16 && sizeof(T) == 4) {
-    // partner lane = lane ^ 16: v_permlane16_swap of x with itself hands
-    // every lane of a 16-lane row pair the even row's value (first result)
-    // and the odd row's (second result); lane p keeps x_0 + (-1)^p x_1
-    const auto sx = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.x), __float_as_uint(x.x),
+    // partner lane = lane ^ 16 (rows of 16 lanes).  v_permlane16_swap(a, b)
+    // gives the even row (a_even, a_odd) and the odd row (b_even, b_odd):
+    // swapping (x.x, x.y) hands the even row both real parts and the odd row
+    // both imaginary parts, each row forms their sum and difference, and
+    // swapping (sum, difference) gives the even row x_0 + x_1 and the odd row
+    // x_0 - x_1.  The two-point DFT is its own inverse up to a factor 2, so
+    // both directions are the same exchange.
+    const auto r1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.x), __float_as_uint(x.y),
                                                      false, false);
-    const auto sy = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.y), __float_as_uint(x.y),
+    const T a = __uint_as_float(r1[0]), b = __uint_as_float(r1[1]);
+    const auto r2 = __builtin_amdgcn_permlane16_swap(__float_as_uint(a + b), __float_as_uint(a - b),
                                                      false, false);
-    const T sg = p & 1 ? T(-1) : T(1);
-    return {__builtin_fmaf(sg, __uint_as_float(sx[1]), __uint_as_float(sx[0])),
-            __builtin_fmaf(sg, __uint_as_float(sy[1]), __uint_as_float(sy[0]))};
+    return {__uint_as_float(r2[0]), __uint_as_float(r2[1])};
   } else if constexpr (P == 2) {
     return bfly_lanes(x, W, p & 1);
   } else {
@@ -807,7 +808,12 @@ __global__ void __launch_bounds__(W * R * P)
   static_assert(P * W <= 64, "the split group must lie in one wavefront");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
-  cpx<T>* tw = buf + pd_size(P * N0 * W);
+  // Two buffers alternate over the six transforms of a tile (3 components
+  // forward, 3 inverse): a buffer is rewritten two transforms after it was
+  // read, and the barrier of the transform between orders the two, so each
+  // transform needs one barrier.
+  constexpr int BS = pd_size(P * N0 * W);
+  cpx<T>* tw = buf + 2 * BS;
   const ModalCoef<T>* coef0 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[0]);
   const ModalCoef<T>* coef1 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[1]);
   const ModalCoef<T>* coef2 = reinterpret_cast<const ModalCoef<T>*>(pp.coef[2]);
@@ -850,7 +856,7 @@ __global__ void __launch_bounds__(W * R * P)
     for (int k = 0; k < R; k++) X[c][k] = src[(int64_t)(R * k) * P * F];
 #pragma unroll
     for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, false>(X[c][k], p);
-    pencil_r_fft<T, R, W, false>(X[c], buf, tw, kb * R, l, w);  // (its barrier also covers tw)
+    pencil_r_fft<T, R, W, false>(X[c], buf + (c & 1) * BS, tw, kb * R, l, w);  // (its barrier also covers tw)
   }
 
   // modal block at k0 = l + R j, k1 = ka + Q kb
@@ -885,7 +891,7 @@ __global__ void __launch_bounds__(W * R * P)
   // inverse: per component, the two passes, the inverse split DFT, the store
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    pencil_r_fft<T, R, W, true>(X[c], buf, tw, kb * R, l, w);
+    pencil_r_fft<T, R, W, true>(X[c], buf + ((c + 1) & 1) * BS, tw, kb * R, l, w);
 #pragma unroll
     for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, true>(X[c][k], p);
     if (wok) {
@@ -935,7 +941,7 @@ static hipError_t launch_pencil_static(cpx<T>* ws, PencilParams pc, int mode, in
     if (!(pf & 8)) {
       constexpr int WR = 32 / P;  // 128-byte rows for P = 2
       auto kp = mode == 0 ? k_pencil_r<T, A0, P, WR, 0> : k_pencil_r<T, A0, P, WR, 1>;
-      const size_t lds = (pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>);
+      const size_t lds = (2 * pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>);
       hipError_t e = set_lds_s(kp, lds);
       if (e != hipSuccess) return e;
       set_tiles(pc, WR);
