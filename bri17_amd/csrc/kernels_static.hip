@@ -139,7 +139,7 @@ __device__ __forceinline__ constexpr int prow(int row) {
 }
 __device__ __forceinline__ constexpr int pcol(int col) { return col + (col >> 5); }
 
-template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF, bool PACK>
 __global__ void __launch_bounds__(Q * M / R1)
     k_plane_fwd_s(const T* __restrict__ u, cpx<T>* __restrict__ ws, cpx<T>* __restrict__ nyq,
                   const cpx<T>* __restrict__ g_twr, const cpx<T>* __restrict__ g_twc, int nitems) {
@@ -254,16 +254,17 @@ __global__ void __launch_bounds__(Q * M / R1)
       }
     }
     // packed column: Y0 = (Z[ka] + conj Z[-ka])/2, YM = -i (Z[ka] - conj Z[-ka])/2
-    if (cs == 0) {
+    // (with PACK the packed column itself is stored, in the k = 0 slot)
+    if (!PACK && cs == 0) {
 #pragma unroll
       for (int b = 0; b < C1 / C2; b++)
 #pragma unroll
         for (int k = 0; k < C2; k++) scr[l + b * TPC + k * C1] = y[b][k];
     }
-    __syncthreads();
+    if (!PACK) __syncthreads();
     // lanes cs = 0 unpack: Y0 replaces the packed value, YM (twiddled) is
     // staged in LDS for the Nyquist region
-    if (cs == 0) {
+    if (!PACK && cs == 0) {
 #pragma unroll
       for (int b = 0; b < C1 / C2; b++)
 #pragma unroll
@@ -289,14 +290,17 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
       for (int k = 0; k < C2; k++) dst[(l + b * TPC + k * C1) * M] = y[b][k];
     __syncthreads();  // LDS is rewritten by the next work item
-    // the item's Nyquist entries: thread tid < Q stores entry tid (scn is
-    // rewritten only after several barriers of the next item)
-    buffer_store(scn[tid & (Q - 1)], rn,
-                 tid < Q ? (uint32_t)(((plane * N1 + p * Q) + tid) * sizeof(cpx<T>)) : kBufferNone);
-    // The next item's loads were issued before this item's C1 + 1 stores:
-    // waiting for them here leaves the stores in flight across the loop head.
+    if constexpr (!PACK) {
+      // the item's Nyquist entries: thread tid < Q stores entry tid (scn is
+      // rewritten only after several barriers of the next item)
+      buffer_store(scn[tid & (Q - 1)], rn,
+                   tid < Q ? (uint32_t)(((plane * N1 + p * Q) + tid) * sizeof(cpx<T>)) : kBufferNone);
+    }
+    // The next item's loads were issued before this item's stores (C1, and
+    // one Nyquist store): waiting for them here leaves the stores in flight
+    // across the loop head.
     static_assert(C1 + 1 < 64, "vmcnt range");
-    if (PF) __builtin_amdgcn_s_waitcnt(vmcnt_wait(C1 + 1));
+    if (PF) __builtin_amdgcn_s_waitcnt(vmcnt_wait(PACK ? C1 : C1 + 1));
   };
   for (; item < nitems; item += gridDim.x) process(item);
 }
@@ -305,7 +309,7 @@ __global__ void __launch_bounds__(Q * M / R1)
 
 // Loads of work item (c, i0, p): column entries (ka = l + kk TPC, cs) into x,
 // and Nyquist entry ka = tid into xn for threads tid < Q (no branch).
-template <typename T, int M, int C1, int Q, int P>
+template <typename T, int M, int C1, int Q, int P, bool PACK>
 __device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>& xn, const cpx<T>* __restrict__ ws,
                                                 __amdgpu_buffer_rsrc_t rn, int item, int tid) {
   constexpr int TPC = Q / C1;
@@ -314,11 +318,12 @@ __device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>& xn, const cpx
   const cpx<T>* src = ws + ((int64_t)plane * (P * Q) + p * Q) * M;
 #pragma unroll
   for (int kk = 0; kk < C1; kk++) x[kk] = src[(l + kk * TPC) * M + cs];
-  xn = buffer_load<T>(rn, tid < Q ? (uint32_t)((plane * (P * Q) + p * Q + tid) * sizeof(cpx<T>))
-                                  : kBufferNone);
+  if constexpr (!PACK)
+    xn = buffer_load<T>(rn, tid < Q ? (uint32_t)((plane * (P * Q) + p * Q + tid) * sizeof(cpx<T>))
+                                    : kBufferNone);
 }
 
-template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF>
+template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF, bool PACK>
 __global__ void __launch_bounds__(Q * M / R1)
     k_plane_inv_s(const cpx<T>* __restrict__ ws, const cpx<T>* __restrict__ nyq,
                   T* __restrict__ out, const cpx<T>* __restrict__ g_twr,
@@ -347,17 +352,19 @@ __global__ void __launch_bounds__(Q * M / R1)
   cpx<T> x[C1], xn;
   int item = blockIdx.x;
   if (PF && item < nitems) {
-    plane_load_cols<T, M, C1, Q, P>(x, xn, ws, rn, item, tid);
+    plane_load_cols<T, M, C1, Q, P, PACK>(x, xn, ws, rn, item, tid);
     __builtin_amdgcn_s_waitcnt(vmcnt_wait(0));
-    if (tid < Q) nin[tid] = xn;
+    if (!PACK && tid < Q) nin[tid] = xn;
   }
   __syncthreads();
   for (; item < nitems; item += gridDim.x) {
     const int tl = opaque(tid);
     if (!PF) {
-      plane_load_cols<T, M, C1, Q, P>(x, xn, ws, rn, item, tl);
-      if (tl < Q) nin[tl] = xn;
-      __syncthreads();
+      plane_load_cols<T, M, C1, Q, P, PACK>(x, xn, ws, rn, item, tl);
+      if (!PACK) {
+        if (tl < Q) nin[tl] = xn;
+        __syncthreads();
+      }
     }
     const int cs = tl % M, l = tl / M;
     const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
@@ -367,8 +374,10 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
     for (int kk = 0; kk < C1; kk++) {
       cpx<T> val = x[kk];
-      const cpx<T> xm = nin[l + kk * TPC];  // same address across the wave
-      val = cs == 0 ? cpx<T>{val.x - xm.y, val.y + xm.x} : val;  // X0 + i XM (both Hermitian)
+      if constexpr (!PACK) {
+        const cpx<T> xm = nin[l + kk * TPC];  // same address across the wave
+        val = cs == 0 ? cpx<T>{val.x - xm.y, val.y + xm.x} : val;  // X0 + i XM (both Hermitian)
+      }
       if (P > 1 && p) val = cmulc(val, twc[p * (l + kk * TPC)]);
       x[kk] = val;
     }
@@ -381,7 +390,7 @@ __global__ void __launch_bounds__(Q * M / R1)
     }
     // prefetch the next work item
     if (PF && item + (int)gridDim.x < nitems)
-      plane_load_cols<T, M, C1, Q, P>(x, xn, ws, rn, item + gridDim.x, tl);
+      plane_load_cols<T, M, C1, Q, P, PACK>(x, xn, ws, rn, item + gridDim.x, tl);
     __syncthreads();
     cpx<T> y[C1 / C2][C2];
     cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
@@ -458,7 +467,7 @@ __global__ void __launch_bounds__(Q * M / R1)
       // waiting for them leaves the stores in flight across the loop head.
       static_assert(R1 < 64, "vmcnt range");
       __builtin_amdgcn_s_waitcnt(vmcnt_wait(R1));
-      if (tl < Q) nin[tl] = xn;
+      if (!PACK && tl < Q) nin[tl] = xn;
     }
     __syncthreads();  // LDS is rewritten by the next work item
   }
@@ -749,63 +758,102 @@ __device__ __forceinline__ void pencil_r_fft(cpx<T> (&v)[R], cpx<T>* buf, const 
   bfly<R, INV>(v);
 }
 
-template <typename T>
-__device__ __forceinline__ cpx<T> shfl_xor_c(cpx<T> x, int m) {
-  return cpx<T>{__shfl_xor(x.x, m), __shfl_xor(x.y, m)};
-}
-
-// P-point DFT (P = 2 or 4) over the split index held by the P lanes p W + w
-// of a group.  Forward: lane p holds x_p on entry and y_kb on exit, kb the
-// bit-reversed p (radix-2 butterfly over the high bit of p, twiddle, radix-2
-// over the low bit); inverse: the conjugate transposed network, from the
-// bit-reversed spectrum back to x_p at lane p (unnormalised).
+// Two-point DFT over the split index held by lanes w and w + 16 of each
+// 32-lane group (P = 2, W = 16).  Forward: lane p holds x_p on entry and
+// y_p on exit; inverse: unnormalised, the same network.
 template <typename T, int P, int W, bool INV>
 __device__ __forceinline__ cpx<T> split_dft(cpx<T> x, int p) {
-  auto bfly_lanes = [&](cpx<T> v, int m, bool second) {
-    const cpx<T> o = shfl_xor_c(v, m);
-    return second ? cpx<T>{o.x - v.x, o.y - v.y} : cpx<T>{v.x + o.x, v.y + o.y};
-  };
-  if constexpr (P == 2 && W == 16 && sizeof(T) == 4) {
-    // partner lane = lane ^ 16 (rows of 16 lanes).  v_permlane16_swap(a, b)
-    // gives the even row (a_even, a_odd) and the odd row (b_even, b_odd):
-    // swapping (x.x, x.y) hands the even row both real parts and the odd row
-    // both imaginary parts, each row forms their sum and difference, and
-    // swapping (sum, difference) gives the even row x_0 + x_1 and the odd row
-    // x_0 - x_1.  The two-point DFT is its own inverse up to a factor 2, so
-    // both directions are the same exchange.
-    const auto r1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.x), __float_as_uint(x.y),
-                                                     false, false);
-    const T a = __uint_as_float(r1[0]), b = __uint_as_float(r1[1]);
-    const auto r2 = __builtin_amdgcn_permlane16_swap(__float_as_uint(a + b), __float_as_uint(a - b),
-                                                     false, false);
-    return {__uint_as_float(r2[0]), __uint_as_float(r2[1])};
-  } else if constexpr (P == 2) {
-    return bfly_lanes(x, W, p & 1);
+  static_assert(P == 2 && W == 16 && sizeof(T) == 4, "lane exchange of 16-lane rows, fp32");
+  (void)p;
+  // v_permlane16_swap(a, b) gives the even row (a_even, a_odd) and the odd
+  // row (b_even, b_odd): swapping (x.x, x.y) hands the even row both real
+  // parts and the odd row both imaginary parts, each row forms their sum and
+  // difference, and swapping (sum, difference) gives the even row x_0 + x_1
+  // and the odd row x_0 - x_1.  The two-point DFT is its own inverse up to a
+  // factor 2, so both directions are the same exchange.
+  const auto r1 = __builtin_amdgcn_permlane16_swap(__float_as_uint(x.x), __float_as_uint(x.y),
+                                                   false, false);
+  const T a = __uint_as_float(r1[0]), b = __uint_as_float(r1[1]);
+  const auto r2 = __builtin_amdgcn_permlane16_swap(__float_as_uint(a + b), __float_as_uint(a - b),
+                                                   false, false);
+  return {__uint_as_float(r2[0]), __uint_as_float(r2[1])};
+}
+
+// Packed DC / Nyquist columns.  Along the last axis the entries k = 0 and
+// k = M of a row are real after the row transform; the plane kernels keep
+// them packed as one complex column Z = X0 + i XM through the middle-axis
+// transform, in the k = 0 slot of each row, so the workspace holds exactly
+// Q M columns per row group and the pencil kernel runs Q M / W tiles.  Over
+// the full middle axis Y0(k1) = (Z(k1) + conj Z(-k1)) / 2 and
+// YM(k1) = -i (Z(k1) - conj Z(-k1)) / 2 are the k = 0 and k = M columns; with
+// the split P = 2 the partner of k1 = ka + Q kb is -k1 = (Q - ka) + Q (1 - kb).
+// The first segment of each row is therefore walked in paired tiles: lanes
+// w < W/2 hold row a, lanes w >= W/2 row Q - a (row Q/2 for a = 0), and the
+// k = 0 lanes exchange Z with their partner lane after the split DFT (lane A
+// keeps Y0(k1), lane B YM(k1) of the same k1) and exchange back before the
+// inverse split DFT (Z(k1) = Y0 + i YM, Z(-k1) = conj Y0 + i conj YM).  For
+// k1 = 0 and k1 = Q (row 0, lane w = 0) Y0 and YM are real sequences along
+// axis 0 packed in one lane: after the axis-0 transform their spectra are
+// separated with the partner frequency -k0 through LDS and receive the
+// blocks of k2 = 0 and k2 = M.
+template <int PM, int W>
+struct PackedTile {
+  int row, k;          // the lane's column: row ka, entry k
+  int k1, kl;          // frequencies of the lane's modal block
+  int xmask;           // partner lane = lane ^ xmask (0: none)
+  bool role_a;         // keeps Y0 (else YM)
+  bool real_pair;      // k1 in {0, Q}, k2 in {0, M} packed in one lane
+};
+template <int PM, int W>
+__device__ __forceinline__ PackedTile<PM, W> packed_tile(int t, int w, int kb, int q) {
+  constexpr int SEG = PM / W;  // tiles per row
+  PackedTile<PM, W> r;
+  const int row = t / SEG, sg = t - (t / SEG) * SEG;
+  r.xmask = 0;
+  r.role_a = true;
+  r.real_pair = false;
+  if (sg) {
+    r.row = row;
+    r.k = sg * W + w;
   } else {
-    static_assert(P == 4, "split of 2 or 4");
-    const bool tw = p == 3;  // w_4^(p0 q0) with p0 = q0 = 1
-    if (!INV) {
-      x = bfly_lanes(x, 2 * W, p & 2);
-      if (tw) x = cpx<T>{x.y, -x.x};  // times -i
-      return bfly_lanes(x, W, p & 1);
+    const int a = row >> 1, h = row & 1;
+    const bool hi = w >= W / 2;
+    r.row = hi ? (a ? q - a : q / 2) : a;
+    r.k = h * (W / 2) + (w & (W / 2 - 1));
+  }
+  r.k1 = r.row + q * kb;
+  r.kl = r.k;
+  if (!sg && !(row & 1) && r.k == 0) {
+    const int a = row >> 1;
+    if (a) {
+      r.xmask = W + W / 2;  // (l, kb, 0) <-> (l, 1 - kb, W/2)
+      r.role_a = w == 0;
+      if (!r.role_a) {
+        r.k1 = a + q * (1 - kb);
+        r.kl = PM;
+      }
+    } else if (w) {
+      r.xmask = W;  // row Q/2: (l, kb) <-> (l, 1 - kb)
+      r.role_a = kb == 0;
+      r.k1 = q / 2;
+      r.kl = kb ? PM : 0;
     } else {
-      x = bfly_lanes(x, W, p & 1);
-      if (tw) x = cpx<T>{-x.y, x.x};  // times +i
-      return bfly_lanes(x, 2 * W, p & 2);
+      r.real_pair = true;
     }
   }
+  return r;
 }
 
-template <int P>
-__device__ __forceinline__ int split_bitrev(int p) {
-  return P == 4 ? ((p & 1) << 1) | (p >> 1) : p;
+template <typename T>
+__device__ __forceinline__ cpx<T> shfl_c(cpx<T> x, int src) {
+  return cpx<T>{__shfl(x.x, src), __shfl(x.y, src)};
 }
 
-template <typename T, int R, int P, int W, int MODE>
+template <typename T, int R, int P, int W, int MODE, int PM>
 __global__ void __launch_bounds__(W * R * P)
     k_pencil_r(cpx<T>* __restrict__ ws, const PencilParams pp) {
   constexpr int N0 = R * R;
-  static_assert(P * W <= 64, "the split group must lie in one wavefront");
+  static_assert(P == 2 && W == 16 && PM % W == 0, "packed columns with the split P = 2");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
   // Two buffers alternate over the six transforms of a tile (3 components
@@ -825,80 +873,136 @@ __global__ void __launch_bounds__(W * R * P)
     const int xx = t % 8, idx = t / 8;
     t = xx * per + (xx < rem ? xx : rem) + idx;
   }
-  const STile st = s_tile(pp, t, W);
-  const int F = st.F;
-  const int f0 = st.f0;
-  const int wv = F - f0 < W ? F - f0 : W;
-  cpx<T>* const wsr = ws + st.off;
+  const int F = pp.f;  // Q M columns per row group (c, i0, p)
+  const int q = pp.q;
   const int w = tid % W;
   const int p = (tid / W) % P;
   const int l = tid / (W * P);
-  const bool wok = w < wv;
-  const int kb = split_bitrev<P>(p);  // split frequency this lane holds after the split DFT
+  const int kb = p;  // split frequency this lane holds after the split DFT
+  const PackedTile<PM, W> pt = packed_tile<PM, W>(t, w, kb, q);
+  const int lane = tid & 63;
+  const int src = lane ^ pt.xmask;
+  const bool paired = t % (PM / W) == 0 && ((t / (PM / W)) & 1) == 0;  // tiles with k = 0 lanes
   copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, W * R * P);
   // (c, i0 = l + R k, p) lies at (l P + p) F + f + (c N0 + R k) P F.  The
-  // lane's pointer is re-derived from an opaque copy of the thread index per
+  // lane's pointer is re-derived from an opaque copy of its column per
   // component, so the compiler does not precompute (and keep) every address.
+  const int fl = pt.row * PM + pt.k;
   auto lane_ptr = [&](int c) {
     const int tl = opaque(tid);
     const int ll = tl / (W * P), lp = (tl / W) % P;
-    const int lw = tl % W;
-    // out-of-range columns of the last tile read a valid address and are never stored
-    const int lf = lw < wv ? f0 + lw : F - 1;
-    return wsr + (int64_t)(ll * P + lp) * F + lf + (int64_t)c * N0 * P * F;
+    return ws + (int64_t)(ll * P + lp) * F + opaque(fl) + (int64_t)c * N0 * P * F;
   };
 
   cpx<T> X[3][R];
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    const cpx<T>* src = lane_ptr(c);
+    const cpx<T>* srcp = lane_ptr(c);
 #pragma unroll
-    for (int k = 0; k < R; k++) X[c][k] = src[(int64_t)(R * k) * P * F];
+    for (int k = 0; k < R; k++) X[c][k] = srcp[(int64_t)(R * k) * P * F];
 #pragma unroll
     for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, false>(X[c][k], p);
+    if (paired) {
+      // unpack: lane A keeps Y0 = (Z_A + conj Z_B) / 2, lane B keeps
+      // YM = -i (Z_A - conj Z_B) / 2 (Z_B = Z(-k1))
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        const cpx<T> o = shfl_c(X[c][k], src), x = X[c][k];
+        if (pt.xmask)
+          X[c][k] = pt.role_a ? cpx<T>{T(0.5) * (x.x + o.x), T(0.5) * (x.y - o.y)}
+                              : cpx<T>{T(0.5) * (o.y + x.y), T(0.5) * (x.x - o.x)};
+      }
+    }
     pencil_r_fft<T, R, W, false>(X[c], buf + (c & 1) * BS, tw, kb * R, l, w);  // (its barrier also covers tw)
   }
 
-  // modal block at k0 = l + R j, k1 = ka + Q kb
-  {
-    int ka, kl;
-    s_freq(pp, st.nyq, f0 + w, ka, kl);
-    if (wok) {
-      const T mu = T(pp.mu), s = T(pp.s);
-      const ModalCoef<T> cc = coef2[kl];
-      const int k1 = ka + pp.q * kb;
-      const ModalCoef<T> b = coef1[k1];
-      const T chichi = b.chi * cc.chi, phichi = b.phi * cc.chi, chiphi = b.chi * cc.phi;
-      const T s_psichi = s * b.psi * cc.chi, s_chipsi = s * b.chi * cc.psi;
-      const T s_psipsi = s * b.psi * cc.psi;
-      const bool kz = k1 == 0 && kl == 0;
+  const T mu = T(pp.mu), s = T(pp.s);
+  cpx<T>* E = buf + BS;  // free until the first inverse transform
+  if (t == 0) {
+    // row 0, k = 0: the spectra of the packed real pair, before the block
+    if (pt.real_pair) {
 #pragma unroll
-      for (int j = 0; j < R; j++) {
-        const int k0 = l + R * j;
-        const ModalCoef<T> a = coef0[k0];
-        const T H0 = a.phi * chichi, H1 = a.chi * phichi, H2 = a.chi * chiphi;
-        const T dg = mu * (H0 + H1 + H2);
-        cpx<T> xx[3] = {X[0][j], X[1][j], X[2][j]};
-        sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
-                            a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
-        X[0][j] = xx[0];
-        X[1][j] = xx[1];
-        X[2][j] = xx[2];
-      }
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int j = 0; j < R; j++) E[(c * 2 + kb) * N0 + l + R * j] = X[c][j];
+    }
+    __syncthreads();
+  }
+
+  // modal block at k0 = l + R j of the lane's (k1, k2)
+  {
+    const ModalCoef<T> cc = coef2[pt.kl];
+    const ModalCoef<T> b = coef1[pt.k1];
+    const T chichi = b.chi * cc.chi, phichi = b.phi * cc.chi, chiphi = b.chi * cc.phi;
+    const T s_psichi = s * b.psi * cc.chi, s_chipsi = s * b.chi * cc.psi;
+    const T s_psipsi = s * b.psi * cc.psi;
+    const bool kz = pt.k1 == 0 && pt.kl == 0;
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const int k0 = l + R * j;
+      const ModalCoef<T> a = coef0[k0];
+      const T H0 = a.phi * chichi, H1 = a.chi * phichi, H2 = a.chi * chiphi;
+      const T dg = mu * (H0 + H1 + H2);
+      cpx<T> xx[3] = {X[0][j], X[1][j], X[2][j]};
+      sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
+                          a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
+      X[0][j] = xx[0];
+      X[1][j] = xx[1];
+      X[2][j] = xx[2];
     }
   }
 
-  // inverse: per component, the two passes, the inverse split DFT, the store
+  if (t == 0) {
+    if (pt.real_pair) {
+      // F = Y0^ + i YM^ with Y0^, YM^ Hermitian in k0: Y0^ = (F + conj F(-k0))/2,
+      // YM^ = -i (F - conj F(-k0))/2; out = B(k2 = 0) Y0^ + i B(k2 = M) YM^
+      auto block = [&](int k0, int kl, bool zero, cpx<T>* xx) {
+        const ModalCoef<T> a = coef0[k0], b = coef1[q * kb], cc = coef2[kl];
+        const T H0 = a.phi * (b.chi * cc.chi), H1 = a.chi * (b.phi * cc.chi);
+        const T H2 = a.chi * (b.chi * cc.phi);
+        const T dg = mu * (H0 + H1 + H2);
+        sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * (s * b.psi * cc.chi),
+                            a.psi * (s * b.chi * cc.psi), a.chi * (s * b.psi * cc.psi), zero, xx);
+      };
+#pragma unroll
+      for (int j = 0; j < R; j++) {
+        const int k0 = l + R * j;
+        cpx<T> y0[3], ym[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const cpx<T> f = E[(c * 2 + kb) * N0 + k0];
+          const cpx<T> g = E[(c * 2 + kb) * N0 + ((N0 - k0) & (N0 - 1))];
+          y0[c] = {T(0.5) * (f.x + g.x), T(0.5) * (f.y - g.y)};
+          ym[c] = {T(0.5) * (f.y + g.y), T(0.5) * (g.x - f.x)};
+        }
+        block(k0, 0, kb == 0 && k0 == 0, y0);
+        block(k0, PM, false, ym);
+#pragma unroll
+        for (int c = 0; c < 3; c++) X[c][j] = {y0[c].x - ym[c].y, y0[c].y + ym[c].x};
+      }
+    }
+    __syncthreads();  // E lies in the buffer of the first inverse transform
+  }
+
+  // inverse: per component, the two passes, the repack, the inverse split
+  // DFT, the store
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     pencil_r_fft<T, R, W, true>(X[c], buf + ((c + 1) & 1) * BS, tw, kb * R, l, w);
+    if (paired) {
+      // lane A: Z = Y0 + i YM; lane B: Z(-k1) = conj Y0 + i conj YM
+#pragma unroll
+      for (int k = 0; k < R; k++) {
+        const cpx<T> o = shfl_c(X[c][k], src), x = X[c][k];
+        if (pt.xmask)
+          X[c][k] = pt.role_a ? cpx<T>{x.x - o.y, x.y + o.x} : cpx<T>{o.x + x.y, x.x - o.y};
+      }
+    }
 #pragma unroll
     for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, true>(X[c][k], p);
-    if (wok) {
-      cpx<T>* dst = lane_ptr(c);
+    cpx<T>* dst = lane_ptr(c);
 #pragma unroll
-      for (int k = 0; k < R; k++) dst[(int64_t)(R * k) * P * F] = X[c][k];
-    }
+    for (int k = 0; k < R; k++) dst[(int64_t)(R * k) * P * F] = X[c][k];
   }
 }
 
@@ -931,20 +1035,28 @@ static void set_tiles(PencilParams& pc, int W) {
   pc.ntiles = (pc.f + W - 1) / W + pc.nyq_tiles;
 }
 
-// Pencil launch of a fixed-size case: the register-block pencil for N0 = R^2
-// (fp32, P = 2 or 4) unless bit 3 of `pf` asks for the all-components-in-LDS
-// pencil; bit 2: persistent variant of the latter.
-template <typename T, int N0, int A0, int B0, int P, int W>
+// The register-block pencil (and with it the packed DC / Nyquist columns)
+// serves fp32 grids with N0 = R^2, split P = 2 and 16-column tiles.
+template <typename T, int N0, int A0, int B0, int P, int M>
+constexpr bool packed_case() {
+  return A0 == B0 && A0 * B0 == N0 && P == 2 && sizeof(T) == 4 && M % 16 == 0;
+}
+
+// Pencil launch of a fixed-size case: the register-block pencil over packed
+// columns where it applies unless bit 3 of `pf` asks for the
+// all-components-in-LDS pencil; bit 2: persistent variant of the latter.
+template <typename T, int N0, int A0, int B0, int P, int W, int M>
 static hipError_t launch_pencil_static(cpx<T>* ws, PencilParams pc, int mode, int pf,
                                        hipStream_t st) {
-  if constexpr (A0 == B0 && A0 * B0 == N0 && (P == 2 || P == 4) && sizeof(T) == 4) {
+  if constexpr (packed_case<T, N0, A0, B0, P, M>()) {
     if (!(pf & 8)) {
-      constexpr int WR = 32 / P;  // 128-byte rows for P = 2
-      auto kp = mode == 0 ? k_pencil_r<T, A0, P, WR, 0> : k_pencil_r<T, A0, P, WR, 1>;
+      constexpr int WR = 16;  // 128-byte rows
+      auto kp = mode == 0 ? k_pencil_r<T, A0, P, WR, 0, M> : k_pencil_r<T, A0, P, WR, 1, M>;
       const size_t lds = (2 * pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>);
       hipError_t e = set_lds_s(kp, lds);
       if (e != hipSuccess) return e;
-      set_tiles(pc, WR);
+      pc.nyq_tiles = 0;
+      pc.ntiles = pc.f / WR;
       hipLaunchKernelGGL(kp, dim3(pc.ntiles), dim3(WR * A0 * P), lds, st, ws, pc);
       return hipGetLastError();
     }
@@ -1013,13 +1125,14 @@ bool static_supported(int dtype, int n0, int n1, int n2, StaticGeom* g) {
 std::string static_describe(int dtype, int n0, int n1, int n2, int pf) {
 #define B17_DESC(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                                  \
   if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                       \
-    const bool reg = A0 == B0 && A0 * B0 == A && (P == 2 || P == 4) && sizeof(T) == 4 && !(pf & 8); \
-    const int wr = reg ? 32 / P : W;                                                              \
+    const bool reg = packed_case<T, A, A0, B0, P, M>() && !(pf & 8);                              \
+    const int wr = reg ? 16 : W;                                                                  \
     return std::string("{\"plane_split\":") + std::to_string(P) +                                 \
            ",\"plane_threads\":" + std::to_string(Q * M / R1) +                                  \
            ",\"plane_forward_persistent\":" + ((pf & 1) ? "true" : "false") +                   \
            ",\"plane_inverse_persistent\":" + ((pf & 2) ? "true" : "false") +                   \
-           ",\"pencil\":\"" + (reg ? "register-block" : "lds") + "\",\"pencil_width\":" +       \
+           ",\"pencil\":\"" + (reg ? "register-block" : "lds") + "\",\"dc_nyquist\":\"" +      \
+           (reg ? "packed" : "separate") + "\",\"pencil_width\":" +                              \
            std::to_string(wr) + ",\"pencil_threads\":" +                                        \
            std::to_string(reg ? wr * A0 * P : 3 * W * (A / A0)) + "}";                            \
   }
@@ -1038,14 +1151,21 @@ hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void
 #define B17_RUN(T, A, B, C, M, R1, R2, Q, C1, C2, P, A0, B0, W)                                   \
   if (dtype == (sizeof(T) == 4 ? 0 : 1) && n0 == A && n1 == B && n2 == C) {                       \
     using PS = PlaneS<T, M, R1, R2, Q, C1, C2, P>;                                                \
-    auto kf = (pf & 1) ? k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, true>                         \
-                       : k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, false>;                        \
-    auto ki = (pf & 2) ? k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, true>                         \
-                       : k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, false>;                        \
+    /* packed DC / Nyquist columns where the register-block pencil runs */                        \
+    constexpr bool PK = packed_case<T, A, A0, B0, P, M>();                                        \
+    const bool pack = PK && !(pf & 8);                                                            \
+    auto kf = pack ? ((pf & 1) ? k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, true, PK>              \
+                               : k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, false, PK>)            \
+                   : ((pf & 1) ? k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, true, false>           \
+                               : k_plane_fwd_s<T, M, R1, R2, Q, C1, C2, P, false, false>);        \
+    auto ki = pack ? ((pf & 2) ? k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, true, PK>              \
+                               : k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, false, PK>)            \
+                   : ((pf & 2) ? k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, true, false>           \
+                               : k_plane_inv_s<T, M, R1, R2, Q, C1, C2, P, false, false>);        \
     e = set_lds_s(kf, PS::lds_fwd());                                                             \
     if (e == hipSuccess) e = set_lds_s(ki, PS::lds_inv());                                        \
     const int nitems = 3 * A * P;                                                                 \
-    /* rows of M entries, then the Nyquist region */                                              \
+    /* rows of M entries, then (unless packed) the Nyquist region */                              \
     PencilParams pc = pc_in;                                                                      \
     pc.rs = M;                                                                                    \
     pc.rsdiv = make_fastdiv(M);                                                                   \
@@ -1062,7 +1182,7 @@ hipError_t launch_static(int dtype, int n0, int n1, int n2, int mode, const void
     }                                                                                             \
     mark(1);                                                                                      \
     if (e == hipSuccess) {                                                                        \
-      e = launch_pencil_static<T, A, A0, B0, P, W>((cpx<T>*)ws, pc, mode, pf, st);               \
+      e = launch_pencil_static<T, A, A0, B0, P, W, M>((cpx<T>*)ws, pc, mode, pf, st);            \
     }                                                                                             \
     mark(2);                                                                                      \
     if (e == hipSuccess) {                                                                        \

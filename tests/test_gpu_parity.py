@@ -219,6 +219,73 @@ def test_fixed_size_launch_variants(device, shape, dtype):
             del op
 
 
+# Single Fourier modes with at least one component N/8 or more: K^(k) is then
+# within a factor of about 7 of its largest value, and fp32 rounding of the
+# transforms (spread over all frequencies, then multiplied by K^ there) stays
+# near 1e-6 of the output.  (For the smoothest modes, e.g. k = (1, 0, 0),
+# that factor reaches ~N^2/10 and every fp32 path, fixed-size or runtime-size,
+# gives errors of 1e-4 to 1e-3 of the max-norm; broadband fields stay near
+# 3e-7.)
+MODES_256 = [(0, 0, 0), (40, 0, 0), (128, 0, 0), (0, 128, 0), (0, 0, 128), (40, 128, 0),
+             (40, 0, 128), (0, 128, 128), (40, 64, 0), (40, 64, 128), (44, 192, 128),
+             (40, 37, 0), (40, 219, 128), (200, 100, 0), (90, 156, 128), (50, 30, 40),
+             (130, 250, 127)]
+
+
+@pytest.mark.parametrize("pf", [3, 11])
+def test_fixed_size_single_modes(device, pf):
+    """256^3 fp32 on single Fourier modes u = Re(a exp(i k.x)): K u equals
+    |h| Re(K^(k) a exp(i k.x)) and the Green solve Re(K^(k)^-1 a exp(i k.x))/|h|
+    (0 at k = 0), with K^(k) from the per-frequency function of the C ABI
+    (bri17.hpp:247-292).  The modes sit on the k2 = 0 and k2 = N2/2 planes,
+    whose columns the plane kernels pack (k1 = 0, N1/4, N1/2, 3N1/4 and
+    partner pairs k1, N1 - k1), and elsewhere; pf 3 runs the register-block
+    pencil over packed columns, pf 11 the LDS pencil over separate Nyquist
+    columns."""
+    import torch
+    from bri17_amd import StiffnessMatrix, pybri17
+    shape, L, mu, nu = (256, 256, 256), (1.0, 1.1, 0.9), 1.2, 0.3
+    h = np.array(L) / np.array(shape)
+    hv = float(np.prod(h))
+    grid = pybri17.CartesianGrid3f64(shape, L)
+    hooke = pybri17.Hooke3f64(mu, nu, grid)
+    op = StiffnessMatrix(shape, L, mu, nu, "float32", tuning=dict(plane_prefetch=pf))
+    rng = np.random.default_rng(11)
+
+    def modal(k):
+        K = np.zeros(9, dtype=np.complex128)
+        hooke.modal_stiffness_matrix(np.array(k, dtype=np.intc), K)
+        return K.reshape(3, 3).real
+
+    ax = [torch.arange(n, device=device, dtype=torch.float64) for n in shape]
+    for k in MODES_256:
+        a = rng.standard_normal(3) + 1j * rng.standard_normal(3)
+        th = 2 * np.pi * (k[0] * ax[0][:, None, None] / shape[0] + k[1] * ax[1][None, :, None] / shape[1]
+                          + k[2] * ax[2][None, None, :] / shape[2])
+        c, s = torch.cos(th), torch.sin(th)
+
+        def field(v):  # Re(v exp(i theta)) per component
+            return torch.stack([float(v[d].real) * c - float(v[d].imag) * s for d in range(3)])
+
+        u = field(a).float()
+        K = modal(k)
+        # k = 0: K u = 0 and the solve drops the mean; errors are measured
+        # against the output of the mode k = (1, 1, 1) of the same amplitude
+        Kr = K if any(k) else modal((1, 1, 1))
+        for mode in ("apply", "solve"):
+            got = getattr(op, mode)(u).double()
+            if mode == "apply":
+                exp = field(hv * (K @ a))
+                ref = np.abs(hv * (Kr @ a)).max()
+            else:
+                exp = field(np.linalg.solve(K, a) / hv) if any(k) else torch.zeros_like(got)
+                ref = np.abs(np.linalg.solve(Kr, a) / hv).max()
+            err = float((got - exp).abs().max()) / ref
+            print(k, mode, "pf", pf, "max err %.2e" % err, flush=True)
+            assert err <= 1e-5, (k, mode)
+        del u, c, s, th
+
+
 @pytest.mark.parametrize("split", [6, 9, 18, 27])
 @pytest.mark.parametrize("mode", ["apply", "solve"])
 def test_middle_axis_splits_without_fused_butterfly(device, oracle, split, mode):
