@@ -873,6 +873,14 @@ __global__ void __launch_bounds__(W * R * P)
     const int xx = t % 8, idx = t / 8;
     t = xx * per + (xx < rem ? xx : rem) + idx;
   }
+  {
+    // The paired tiles of even rows (tile 2 a SEG, a < Q/2) carry the lane
+    // exchanges and take longer: they are dispatched first, so that the
+    // workgroup slots they hold take fewer of the other tiles.
+    constexpr int G = 2 * (PM / W);  // one exchange tile in every G tiles
+    const int nx = (int)gridDim.x / G;
+    t = t < nx ? t * G : ((t - nx) / (G - 1)) * G + (t - nx) % (G - 1) + 1;
+  }
   const int F = pp.f;  // Q M columns per row group (c, i0, p)
   const int q = pp.q;
   const int w = tid % W;
@@ -904,13 +912,21 @@ __global__ void __launch_bounds__(W * R * P)
     for (int k = 0; k < R; k++) X[c][k] = split_dft<T, P, W, false>(X[c][k], p);
     if (paired) {
       // unpack: lane A keeps Y0 = (Z_A + conj Z_B) / 2, lane B keeps
-      // YM = -i (Z_A - conj Z_B) / 2 (Z_B = Z(-k1))
+      // YM = -i (Z_A - conj Z_B) / 2 (Z_B = Z(-k1)); exchanges in groups of
+      // four, then selects (no divergent branch per value)
 #pragma unroll
-      for (int k = 0; k < R; k++) {
-        const cpx<T> o = shfl_c(X[c][k], src), x = X[c][k];
-        if (pt.xmask)
-          X[c][k] = pt.role_a ? cpx<T>{T(0.5) * (x.x + o.x), T(0.5) * (x.y - o.y)}
-                              : cpx<T>{T(0.5) * (o.y + x.y), T(0.5) * (x.x - o.x)};
+      for (int g = 0; g < R; g += 4) {
+        cpx<T> o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = shfl_c(X[c][g + k], src);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const cpx<T> x = X[c][g + k];
+          const cpx<T> ya = {T(0.5) * (x.x + o[k].x), T(0.5) * (x.y - o[k].y)};
+          const cpx<T> yb = {T(0.5) * (o[k].y + x.y), T(0.5) * (x.x - o[k].x)};
+          const cpx<T> y = pt.role_a ? ya : yb;
+          X[c][g + k] = pt.xmask ? y : x;
+        }
       }
     }
     pencil_r_fft<T, R, W, false>(X[c], buf + (c & 1) * BS, tw, kb * R, l, w);  // (its barrier also covers tw)
@@ -992,10 +1008,17 @@ __global__ void __launch_bounds__(W * R * P)
     if (paired) {
       // lane A: Z = Y0 + i YM; lane B: Z(-k1) = conj Y0 + i conj YM
 #pragma unroll
-      for (int k = 0; k < R; k++) {
-        const cpx<T> o = shfl_c(X[c][k], src), x = X[c][k];
-        if (pt.xmask)
-          X[c][k] = pt.role_a ? cpx<T>{x.x - o.y, x.y + o.x} : cpx<T>{o.x + x.y, x.x - o.y};
+      for (int g = 0; g < R; g += 4) {
+        cpx<T> o[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[k] = shfl_c(X[c][g + k], src);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const cpx<T> x = X[c][g + k];
+          const cpx<T> za = {x.x - o[k].y, x.y + o[k].x}, zb = {o[k].x + x.y, x.x - o[k].y};
+          const cpx<T> z = pt.role_a ? za : zb;
+          X[c][g + k] = pt.xmask ? z : x;
+        }
       }
     }
 #pragma unroll
