@@ -123,8 +123,14 @@ __device__ __forceinline__ void plane_load_rows(cpx<T>* v, const T* __restrict__
   const int q = tid / TPR, j = tid - (tid / TPR) * TPR;
   const cpx<T>* src =
       reinterpret_cast<const cpx<T>*>(u + ((int64_t)plane * (P * Q) + q * P + p) * (2 * M));
+  // The field is read once: non-temporal loads leave the last-level cache
+  // (256 MB) to the half-spectrum workspace, which the pencil and inverse
+  // kernels then read from there.
 #pragma unroll
-  for (int k = 0; k < R1; k++) v[k] = src[j + k * TPR];
+  for (int k = 0; k < R1; k++) {
+    const T* sp = reinterpret_cast<const T*>(src + j + k * TPR);
+    v[k] = cpx<T>{__builtin_nontemporal_load(sp), __builtin_nontemporal_load(sp + 1)};
+  }
 }
 
 // LDS index of (row, col) in rows of RS entries padded by one entry every 32
