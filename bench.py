@@ -30,13 +30,15 @@ KERNELS = ("plane_r2c", "pencil_block", "plane_c2r")
 def _kernel_bytes(shape, esz):
     """Algorithmic HBM bytes per launch of each kernel: read its input once,
     write its output once.  Real field: DIM*|N|*esz; half-spectrum
-    workspace: DIM*(|N|/NL)*(NL/2+1)*2*esz."""
+    workspace: the R2C output of the real field with its two real columns
+    (k = 0 and k = NL/2) packed into one complex column, DIM*|N|/2 complex
+    values, the same bytes as the field."""
     D = len(shape)
     N = 1
     for n in shape:
         N *= n
     field = D * N * esz
-    half = D * (N // shape[-1]) * (shape[-1] // 2 + 1) * 2 * esz
+    half = field
     return {"plane_r2c": field + half, "pencil_block": 2 * half, "plane_c2r": half + field}
 
 
@@ -193,13 +195,16 @@ def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
         for m in ("apply", "solve"):
             submit((n, m, 1.0, 0.3))
     srv.drain()
-    # capacity: closed-loop burst
-    burst = draw(n_capacity)
-    t0 = time.perf_counter()
-    reqs = [submit(q) for q in burst]
-    for r in reqs:
-        r.wait()
-    cap = n_capacity / (time.perf_counter() - t0)
+    # capacity: closed-loop bursts (the faster of two: the first also grows
+    # the server's buffer pool)
+    cap = 0.0
+    for _ in range(2):
+        burst = draw(n_capacity)
+        t0 = time.perf_counter()
+        reqs = [submit(q) for q in burst]
+        for r in reqs:
+            r.wait()
+        cap = max(cap, n_capacity / (time.perf_counter() - t0))
     # open loop: Poisson arrivals at load * capacity
     rate = load * cap
     arr = poisson_arrivals(rate, n_requests, seed=seed + 1)
