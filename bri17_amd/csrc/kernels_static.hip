@@ -133,17 +133,24 @@ __device__ __forceinline__ void plane_load_rows(cpx<T>* v, const T* __restrict__
   }
 }
 
-// LDS index of (row, col) in rows of RS entries padded by one entry every 32
-// (pd(row RS + col) for RS a multiple of 32).  Callers split it into a lane
-// part and a compile-time part so that the latter becomes an instruction
-// offset: row and col terms are independent, and col >> 5 splits when the
-// constant part of col is a multiple of 32 or cannot carry into bit 5.
+// LDS index of (row, col) in the plane kernels' rows of RS entries, padded by
+// one entry every 16 (a padded row of 128 entries is 136 = 8 mod 16 entries
+// long).  With 8-byte entries a quarter wave's 16 lanes are conflict-free
+// when their indices differ mod 16: the row passes (lanes j of rows q, q+1
+// at columns 16 j + k, or j + 8 b + 16 k) and the column passes (16
+// consecutive columns from a multiple of 16) all are.  Callers split the
+// index into a lane part and a compile-time part so that the latter becomes
+// an instruction offset: row and col terms are independent, and col >> 4
+// splits when the constant part of col is a multiple of 16 or cannot carry
+// into bit 4.
 template <int RS>
 __device__ __forceinline__ constexpr int prow(int row) {
-  static_assert(RS % 32 == 0, "padded rows of 32-entry multiples");
-  return row * (RS + RS / 32);
+  static_assert(RS % 16 == 0, "padded rows of 16-entry multiples");
+  return row * (RS + RS / 16);
 }
-__device__ __forceinline__ constexpr int pcol(int col) { return col + (col >> 5); }
+__device__ __forceinline__ constexpr int pcol(int col) { return col + (col >> 4); }
+// entries of Q padded rows of RS entries (one spare entry past the end)
+__host__ __device__ constexpr int prow_size(int q, int rs) { return q * (rs + rs / 16) + 1; }
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P, bool PF, bool PACK>
 __global__ void __launch_bounds__(Q * M / R1)
@@ -156,11 +163,11 @@ __global__ void __launch_bounds__(Q * M / R1)
   constexpr int RS = M;  // row stride (before padding)
   static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
-  static_assert(32 % R1 == 0 && M % 32 == 0, "LDS offsets split into lane + constant parts");
+  static_assert(R1 == 16 && M % 16 == 0, "LDS offsets split into lane + constant parts");
   static_assert((M & (M - 1)) == 0 && (Q & (Q - 1)) == 0, "power-of-two plane sides");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
-  cpx<T>* twr = buf + pd_size(Q * RS);   // w_{2M}^j, j < 2M
+  cpx<T>* twr = buf + prow_size(Q, RS);  // w_{2M}^j, j < 2M
   cpx<T>* twc = twr + 2 * M;             // w_{N1}^j, j < N1
   cpx<T>* scr = twc + N1;                // packed-column scratch, Q entries
   cpx<T>* scn = scr + Q;                 // Nyquist entries of the item, Q entries
@@ -340,11 +347,11 @@ __global__ void __launch_bounds__(Q * M / R1)
   constexpr int N1 = P * Q;
   static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
-  static_assert(32 % R1 == 0 && 32 % TPR == 0 && M % 32 == 0,
+  static_assert(R1 == 16 && 16 % TPR == 0 && M % 16 == 0,
                 "LDS offsets split into lane + constant parts");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);  // rows of M entries (padded)
-  cpx<T>* nyl = buf + pd_size(Q * M);              // X[q][M] of every row
+  cpx<T>* nyl = buf + prow_size(Q, M);             // X[q][M] of every row
   cpx<T>* twr = nyl + Q;
   cpx<T>* twc = twr + 2 * M;
   cpx<T>* nin = twc + N1;                          // the item's Nyquist entries
@@ -1047,8 +1054,8 @@ static hipError_t set_lds_s(K kernel, size_t lds) {
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
 struct PlaneS {
-  static size_t lds_fwd() { return (pd_size(Q * M) + 2 * M + P * Q + 2 * Q) * sizeof(cpx<T>); }
-  static size_t lds_inv() { return (pd_size(Q * M) + Q + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
+  static size_t lds_fwd() { return (prow_size(Q, M) + 2 * M + P * Q + 2 * Q) * sizeof(cpx<T>); }
+  static size_t lds_inv() { return (prow_size(Q, M) + Q + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
   static constexpr int threads() { return Q * M / R1; }
 };
 
