@@ -171,11 +171,13 @@ __global__ void __launch_bounds__(Q * M / R1)
   cpx<T>* twc = twr + 2 * M;             // w_{N1}^j, j < N1
   cpx<T>* scr = twc + N1;                // packed-column scratch, Q entries
   cpx<T>* scn = scr + Q;                 // Nyquist entries of the item, Q entries
+  cpx<T>* tw2 = scn + Q;                 // row pass 2 twiddles w_M^(jj k) at [k R1 + jj]
   const int tid = threadIdx.x;
   const __amdgpu_buffer_rsrc_t rn = buffer_rsrc(nyq, (uint32_t)(nitems * Q * sizeof(cpx<T>)));
 
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
+  for (int i = tid; i < R2 * R1; i += NT) tw2[i] = g_twr[2 * (i % R1) * (i / R1)];
 
   cpx<T> v[R1];
   int item = blockIdx.x;
@@ -211,11 +213,11 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
       for (int k = 0; k < R2; k++) {
         w[b][k] = rp[b * TPR + pcol(k * R1)];
-        if (k) w[b][k] = cmul(w[b][k], twr[2 * jj * k]);
+        if (k) w[b][k] = cmul(w[b][k], tw2[k * R1 + jj]);
       }
       bfly<R2, false>(w[b]);
     }
-    __syncthreads();
+    // in place: a thread writes back exactly the entries it read (no barrier)
 #pragma unroll
     for (int b = 0; b < R1 / R2; b++)
 #pragma unroll
@@ -355,12 +357,14 @@ __global__ void __launch_bounds__(Q * M / R1)
   cpx<T>* twr = nyl + Q;
   cpx<T>* twc = twr + 2 * M;
   cpx<T>* nin = twc + N1;                          // the item's Nyquist entries
+  cpx<T>* tw2 = nin + Q;                           // row pass 2 twiddles, [k R1 + jj]
   const int tid = threadIdx.x;
   const __amdgpu_buffer_rsrc_t rn =
       buffer_rsrc(const_cast<cpx<T>*>(nyq), (uint32_t)(nitems * Q * sizeof(cpx<T>)));
 
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
+  for (int i = tid; i < R2 * R1; i += NT) tw2[i] = g_twr[2 * (i % R1) * (i / R1)];
 
   cpx<T> x[C1], xn;
   int item = blockIdx.x;
@@ -417,8 +421,8 @@ __global__ void __launch_bounds__(Q * M / R1)
       }
       bfly<C2, true>(y[b]);
     }
-    __syncthreads();
-    // rows q = jj + C1 k, natural order; unpack columns 0 and M
+    // rows q = jj + C1 k, natural order, written back in place (a thread
+    // rewrites exactly the entries it read: no barrier); unpack columns 0 and M
 #pragma unroll
     for (int b = 0; b < C1 / C2; b++)
 #pragma unroll
@@ -466,7 +470,7 @@ __global__ void __launch_bounds__(Q * M / R1)
 #pragma unroll
       for (int k = 0; k < R2; k++) {
         w[b][k] = rp[b * TPR + pcol(k * R1)];
-        if (k) w[b][k] = cmulc(w[b][k], twr[2 * jj * k]);
+        if (k) w[b][k] = cmulc(w[b][k], tw2[k * R1 + jj]);
       }
       bfly<R2, true>(w[b]);
     }
@@ -1054,8 +1058,12 @@ static hipError_t set_lds_s(K kernel, size_t lds) {
 
 template <typename T, int M, int R1, int R2, int Q, int C1, int C2, int P>
 struct PlaneS {
-  static size_t lds_fwd() { return (prow_size(Q, M) + 2 * M + P * Q + 2 * Q) * sizeof(cpx<T>); }
-  static size_t lds_inv() { return (prow_size(Q, M) + Q + 2 * M + P * Q + Q) * sizeof(cpx<T>); }
+  static size_t lds_fwd() {
+    return (prow_size(Q, M) + 2 * M + P * Q + 2 * Q + R1 * R2) * sizeof(cpx<T>);
+  }
+  static size_t lds_inv() {
+    return (prow_size(Q, M) + Q + 2 * M + P * Q + Q + R1 * R2) * sizeof(cpx<T>);
+  }
   static constexpr int threads() { return Q * M / R1; }
 };
 
