@@ -14,6 +14,22 @@
 
 namespace b17 {
 
+// 1/d for the block inverse.  fp32: the hardware reciprocal (v_rcp_f32,
+// 1 ulp) refined by one Newton-Raphson step, e = 1 - d r, r' = r + r e (both
+// fused), which leaves an error within about 1 ulp without the
+// scale/fixup sequence of an IEEE division.  d is the determinant of an SPD
+// block away from k = 0 (normal range; k = 0 is handled before).  fp64: the
+// division.
+template <typename T>
+__device__ __forceinline__ T block_recip(T d) {
+  if constexpr (sizeof(T) == 4) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return __builtin_fmaf(r, __builtin_fmaf(-d, r, 1.0f), r);
+  } else {
+    return T(1) / d;
+  }
+}
+
 // y = K x (MODE 0) or y = K^-1 x (MODE 1, y = 0 when `null`) for the real
 // symmetric 3x3 block K = [[K00,K01,K02],[K01,K11,K12],[K02,K12,K22]].
 template <typename T, int MODE>
@@ -38,7 +54,7 @@ __device__ __forceinline__ void sym3_apply(T K00, T K11, T K22, T K01, T K02, T 
     const T A11 = K00 * K22 - K02 * K02;
     const T A12 = K01 * K02 - K00 * K12;
     const T A22 = K00 * K11 - K01 * K01;
-    const T r = T(1) / (K00 * A00 + K01 * A01 + K02 * A02);
+    const T r = block_recip(K00 * A00 + K01 * A01 + K02 * A02);
     const cpx<T> y0 = (x[0] * A00 + x[1] * A01 + x[2] * A02) * r;
     const cpx<T> y1 = (x[0] * A01 + x[1] * A11 + x[2] * A12) * r;
     const cpx<T> y2 = (x[0] * A02 + x[1] * A12 + x[2] * A22) * r;
@@ -60,7 +76,7 @@ __device__ __forceinline__ void sym2_apply(T K00, T K11, T K01, bool null, cpx<T
       x[0] = x[1] = cpx<T>{T(0), T(0)};
       return;
     }
-    const T r = T(1) / (K00 * K11 - K01 * K01);
+    const T r = block_recip(K00 * K11 - K01 * K01);
     const cpx<T> y0 = (x[0] * K11 - x[1] * K01) * r;
     const cpx<T> y1 = (x[1] * K00 - x[0] * K01) * r;
     x[0] = y0;
