@@ -26,19 +26,23 @@ __device__ __forceinline__ int pd(int i) { return i + (i >> 5); }
 __host__ __device__ constexpr int pd_size(int n) { return n + (n >> 5) + 1; }
 
 template <typename T>
-__device__ __forceinline__ cpx<T> s_post_r2c(cpx<T> A, cpx<T> B, cpx<T> w) {
-  // X[k] = (A + conj B)/2 - i w (A - conj B)/2 with A = Z[k], B = Z[M-k]
-  const cpx<T> e = {T(0.5) * (A.x + B.x), T(0.5) * (A.y - B.y)};
-  const cpx<T> d = {T(0.5) * (A.x - B.x), T(0.5) * (A.y + B.y)};
-  return e + cmul(w, cpx<T>{d.y, -d.x});
+__device__ __forceinline__ T fmat(T a, T b, T c) {
+  if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, c);
+  else return __builtin_fma(a, b, c);
 }
 
+// The two half-length real-transform splits with w = w_{2M}^k:
+//   R2C: X[k] = (A + conj B)/2 - i w (A - conj B)/2,  A = Z[k], B = Z[M-k]
+//   C2R: Z[k] = (A + conj B) + i conj(w) (A - conj B), A = X[k], B = X[M-k]
+// regrouped per input, four fused operations per output part, with the
+// per-entry constants p = 1 + w.y, m = 1 - w.y, h = w.x (halved for the R2C,
+// which the forward plane kernel writes out per component):
+//   R2C: X.x = A.x p + B.x m + (A.y + B.y) h,  X.y = A.y p - B.y m - (A.x - B.x) h
+//   C2R: Z.x = A.x p + B.x m - (A.y + B.y) h,  Z.y = A.y p - B.y m + (A.x - B.x) h
 template <typename T>
-__device__ __forceinline__ cpx<T> s_pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
-  // Z[k] = (A + conj B) + i conj(w) (A - conj B) with A = X[k], B = X[M-k]
-  const cpx<T> e = {A.x + B.x, A.y - B.y};
-  const cpx<T> o = cmulc(cpx<T>{A.x - B.x, A.y + B.y}, w);
-  return {e.x - o.y, e.y + o.x};
+__device__ __forceinline__ cpx<T> s_pre_c2r_pmh(cpx<T> A, cpx<T> B, T p, T m, T h) {
+  return {fmat(A.x, p, fmat(B.x, m, fmat(-A.y, h, -(B.y * h)))),
+          fmat(A.y, p, fmat(-B.y, m, fmat(A.x, h, -(B.x * h))))};
 }
 
 // A copy of v the optimiser cannot see through: indices derived from it are
@@ -233,6 +237,7 @@ __global__ void __launch_bounds__(Q * M / R1)
       const cpx<T>* cp = buf + prow<RS>(l) + pcol(cs);
       const cpx<T>* cm = buf + prow<RS>(l) + pcol((M - cs) & (M - 1));
       const cpx<T> wc = twr[cs];
+      const T wp = T(0.5) * (T(1) + wc.y), wm = T(0.5) * (T(1) - wc.y), wh = T(0.5) * wc.x;
       cpx<T> A[C1], B[C1];
 #pragma unroll
       for (int kk = 0; kk < C1; kk++) {
@@ -241,8 +246,12 @@ __global__ void __launch_bounds__(Q * M / R1)
       }
 #pragma unroll
       for (int kk = 0; kk < C1; kk++) {
-        x[kk] = s_post_r2c(A[kk], B[kk], wc);
-        x[kk].y = cs == 0 ? A[kk].x - A[kk].y : x[kk].y;
+        const cpx<T> a = A[kk], b = B[kk];
+        // written per component: returned as a pair, the same operations
+        // were scheduled into 128 VGPRs instead of 98
+        x[kk].x = fmat(a.x, wp, fmat(b.x, wm, fmat(a.y, wh, b.y * wh)));
+        x[kk].y = fmat(a.y, wp, fmat(-b.y, wm, fmat(-a.x, wh, b.x * wh)));
+        x[kk].y = cs == 0 ? a.x - a.y : x[kk].y;
       }
     }
     bfly<C1, false>(x);
@@ -354,15 +363,19 @@ __global__ void __launch_bounds__(Q * M / R1)
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);  // rows of M entries (padded)
   cpx<T>* nyl = buf + prow_size(Q, M);             // X[q][M] of every row
-  cpx<T>* twr = nyl + Q;
-  cpx<T>* twc = twr + 2 * M;
+  cpx<T>* tpm = nyl + Q;                           // (1 + w.y, 1 - w.y), (w.x, 0) of w_{2M}^mm
+  cpx<T>* twc = tpm + 2 * M;
   cpx<T>* nin = twc + N1;                          // the item's Nyquist entries
   cpx<T>* tw2 = nin + Q;                           // row pass 2 twiddles, [k R1 + jj]
   const int tid = threadIdx.x;
   const __amdgpu_buffer_rsrc_t rn =
       buffer_rsrc(const_cast<cpx<T>*>(nyq), (uint32_t)(nitems * Q * sizeof(cpx<T>)));
 
-  copy_table(twr, g_twr, 2 * M, tid, NT);
+  for (int i = tid; i < M; i += NT) {
+    const cpx<T> w = g_twr[i];
+    tpm[2 * i] = {T(1) + w.y, T(1) - w.y};
+    tpm[2 * i + 1] = {w.x, T(0)};
+  }
   copy_table(twc, g_twc, N1, tid, NT);
   for (int i = tid; i < R2 * R1; i += NT) tw2[i] = g_twr[2 * (i % R1) * (i / R1)];
 
@@ -451,7 +464,8 @@ __global__ void __launch_bounds__(Q * M / R1)
         const cpx<T> A = rp[j + pcol(k * TPR)];
         const cpx<T>* bp = rp + pcol(M - mm);
         if (k == 0) bp = j == 0 ? nyl + q : bp;
-        v[k] = s_pre_c2r(A, *bp, twr[mm]);
+        const cpx<T> pm = tpm[2 * mm], h = tpm[2 * mm + 1];
+        v[k] = s_pre_c2r_pmh(A, *bp, pm.x, pm.y, h.x);
       }
     }
     bfly<R1, true>(v);
