@@ -45,6 +45,17 @@ __device__ __forceinline__ cpx<T> s_pre_c2r_pmh(cpx<T> A, cpx<T> B, T p, T m, T 
           fmat(A.y, p, fmat(-B.y, m, fmat(A.x, h, -(B.x * h))))};
 }
 
+// Orders one wave's LDS accesses around it for the compiler (a wavefront-
+// scope fence and wave barrier; the hardware executes a wave's LDS
+// instructions in order).  An exchange whose writers and readers all lie in
+// one wave needs no workgroup barrier: the plane kernels' row exchanges,
+// where the M/R1 threads of a row are consecutive lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // A copy of v the optimiser cannot see through: indices derived from it are
 // recomputed inside a persistent loop instead of being hoisted into hundreds
 // of live registers.
@@ -168,6 +179,7 @@ __global__ void __launch_bounds__(Q * M / R1)
   static_assert(R1 * R2 == M && C1 * C2 == Q && R1 == C1, "plane geometry");
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
   static_assert(R1 == 16 && M % 16 == 0, "LDS offsets split into lane + constant parts");
+  static_assert(64 % TPR == 0, "the threads of a row lie in one wave");
   static_assert((M & (M - 1)) == 0 && (Q & (Q - 1)) == 0, "power-of-two plane sides");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
@@ -182,6 +194,7 @@ __global__ void __launch_bounds__(Q * M / R1)
   copy_table(twr, g_twr, 2 * M, tid, NT);
   copy_table(twc, g_twc, N1, tid, NT);
   for (int i = tid; i < R2 * R1; i += NT) tw2[i] = g_twr[2 * (i % R1) * (i / R1)];
+  __syncthreads();
 
   cpx<T> v[R1];
   int item = blockIdx.x;
@@ -206,7 +219,8 @@ __global__ void __launch_bounds__(Q * M / R1)
     // prefetch the next work item into the now free registers
     if (PF && item + (int)gridDim.x < nitems)
       plane_load_rows<T, M, R1, Q, P>(v, u, item + gridDim.x, tl);
-    __syncthreads();
+    // the row's TPR threads are lanes of one wave: no workgroup barrier
+    wave_lds_sync();
     // pass 2: R1/R2 butterflies per thread, natural-order output;
     // (q, jj + k R1) with jj = j + b TPR < R1
     cpx<T> w[R1 / R2][R2];
@@ -360,6 +374,7 @@ __global__ void __launch_bounds__(Q * M / R1)
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
   static_assert(R1 == 16 && 16 % TPR == 0 && M % 16 == 0,
                 "LDS offsets split into lane + constant parts");
+  static_assert(64 % TPR == 0, "the threads of a row lie in one wave");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);  // rows of M entries (padded)
   cpx<T>* nyl = buf + prow_size(Q, M);             // X[q][M] of every row
@@ -469,13 +484,14 @@ __global__ void __launch_bounds__(Q * M / R1)
       }
     }
     bfly<R1, true>(v);
-    __syncthreads();
+    // row q's entries are read and rewritten by the TPR lanes of one wave
+    wave_lds_sync();
     {
       cpx<T>* bp = buf + prow<M>(q) + pcol(j * R1);
 #pragma unroll
       for (int k = 0; k < R1; k++) bp[k] = v[k];
     }
-    __syncthreads();
+    wave_lds_sync();
     cpx<T> w[R1 / R2][R2];
     const cpx<T>* rp = buf + prow<M>(q) + j;
 #pragma unroll
