@@ -939,6 +939,10 @@ __global__ void __launch_bounds__(W * R * P)
   const int src = lane ^ pt.xmask;
   const bool paired = t % (PM / W) == 0 && ((t / (PM / W)) & 1) == 0;  // tiles with k = 0 lanes
   copy_table(tw, reinterpret_cast<const cpx<T>*>(pp.tw0), N0, tid, W * R * P);
+  // axis-0 block coefficients in LDS (read once per frequency, 16 per
+  // thread: LDS latency instead of a chain of L2 loads)
+  ModalCoef<T>* cf0 = reinterpret_cast<ModalCoef<T>*>(tw + N0);
+  for (int i = tid; i < N0; i += W * R * P) cf0[i] = coef0[i];
   // (c, i0 = l + R k, p) lies at (l P + p) F + f + (c N0 + R k) P F.  The
   // lane's pointer is re-derived from an opaque copy of its column per
   // component, so the compiler does not precompute (and keep) every address.
@@ -1003,7 +1007,7 @@ __global__ void __launch_bounds__(W * R * P)
 #pragma unroll
     for (int j = 0; j < R; j++) {
       const int k0 = l + R * j;
-      const ModalCoef<T> a = coef0[k0];
+      const ModalCoef<T> a = cf0[k0];
       const T H0 = a.phi * chichi, H1 = a.chi * phichi, H2 = a.chi * chiphi;
       const T dg = mu * (H0 + H1 + H2);
       cpx<T> xx[3] = {X[0][j], X[1][j], X[2][j]};
@@ -1126,7 +1130,7 @@ static hipError_t launch_pencil_static(cpx<T>* ws, PencilParams pc, int mode, in
     if (!(pf & 8)) {
       constexpr int WR = 16;  // 128-byte rows
       auto kp = mode == 0 ? k_pencil_r<T, A0, P, WR, 0, M> : k_pencil_r<T, A0, P, WR, 1, M>;
-      const size_t lds = (2 * pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>);
+      const size_t lds = (2 * pd_size(P * N0 * WR) + N0) * sizeof(cpx<T>) + N0 * sizeof(ModalCoef<T>);
       hipError_t e = set_lds_s(kp, lds);
       if (e != hipSuccess) return e;
       pc.nyq_tiles = 0;
