@@ -48,12 +48,30 @@ __device__ __forceinline__ cpx<T> s_pre_c2r_pmh(cpx<T> A, cpx<T> B, T p, T m, T 
 // Orders one wave's LDS accesses around it for the compiler (a wavefront-
 // scope fence and wave barrier; the hardware executes a wave's LDS
 // instructions in order).  An exchange whose writers and readers all lie in
-// one wave needs no workgroup barrier: the plane kernels' row exchanges,
-// where the M/R1 threads of a row are consecutive lanes of one wave.
+// one wave needs no workgroup barrier: the plane kernels' row exchanges
+// (the M/R1 threads of a row are consecutive lanes of one wave) and column
+// exchanges (see col_cs).
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Column-stage thread mapping of the plane kernels: the TPC threads of a
+// column are lanes of one wave (64 / TPC consecutive columns per wave, column
+// fastest), so the column exchange is ordered per wave.  A global access of
+// a wave then covers TPC rows of 64 / TPC columns (64-byte segments for fp32
+// at TPC = 8), whose other halves the neighbouring wave of the same CU
+// accesses at the same time.
+template <int TPC>
+__device__ __forceinline__ int col_cs(int tl) {
+  constexpr int CPW = 64 / TPC;
+  return (tl / 64) * CPW + (tl % CPW);
+}
+template <int TPC>
+__device__ __forceinline__ int col_l(int tl) {
+  constexpr int CPW = 64 / TPC;
+  return (tl % 64) / CPW;
 }
 
 // A copy of v the optimiser cannot see through: indices derived from it are
@@ -180,6 +198,8 @@ __global__ void __launch_bounds__(Q * M / R1)
   static_assert(R1 % R2 == 0 && C1 % C2 == 0, "second pass must divide the first");
   static_assert(R1 == 16 && M % 16 == 0, "LDS offsets split into lane + constant parts");
   static_assert(64 % TPR == 0, "the threads of a row lie in one wave");
+  static_assert(64 % TPC == 0 && M % (64 / TPC) == 0 && NT % 64 == 0,
+                "the threads of a column lie in one wave");
   static_assert((M & (M - 1)) == 0 && (Q & (Q - 1)) == 0, "power-of-two plane sides");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);
@@ -206,7 +226,7 @@ __global__ void __launch_bounds__(Q * M / R1)
     const int tl = opaque(tid);
     if (!PF) plane_load_rows<T, M, R1, Q, P>(v, u, item, tl);
     const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
-    const int cs = tl % M, l = tl / M;
+    const int cs = col_cs<TPC>(tl), l = col_l<TPC>(tl);
     const int plane = item / P;
     const int p = item - plane * P;
     // ---- rows: pass 1 on the loaded registers; (q, j R1 + k), k < R1
@@ -270,22 +290,28 @@ __global__ void __launch_bounds__(Q * M / R1)
     }
     bfly<C1, false>(x);
     __syncthreads();
-    // column exchange layout: element idx of column cs at row idx (stride M)
+    // column exchange layout: element m of sub-sequence l of column cs at row
+    // l C1 + (m ^ (l & 1)) (the swizzle keeps both the writes and the reads
+    // of a quarter wave on distinct banks)
     {
-      cpx<T>* cp = buf + prow<M>(l * C1) + pcol(cs);
+      const int lb = l & 1;
+      cpx<T>* ce = buf + prow<M>(l * C1 + lb) + pcol(cs);
+      cpx<T>* co = buf + prow<M>(l * C1 - lb) + pcol(cs);
 #pragma unroll
-      for (int kk = 0; kk < C1; kk++) cp[prow<M>(kk)] = x[kk];
+      for (int kk = 0; kk < C1; kk++) ((kk & 1) ? co : ce)[prow<M>(kk)] = x[kk];
     }
-    __syncthreads();
+    // the column's TPC threads are lanes of one wave
+    wave_lds_sync();
     cpx<T> y[C1 / C2][C2];
     {
-      const cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
+      const cpx<T>* ce = buf + prow<M>(l) + pcol(cs);
+      const cpx<T>* co = buf + prow<M>(l ^ 1) + pcol(cs);
 #pragma unroll
       for (int b = 0; b < C1 / C2; b++) {
         const int jj = l + b * TPC;
 #pragma unroll
         for (int k = 0; k < C2; k++) {
-          y[b][k] = cp[prow<M>(b * TPC + k * C1)];
+          y[b][k] = ((k & 1) ? co : ce)[prow<M>(b * TPC + k * C1)];
           if (k) y[b][k] = cmul(y[b][k], twc[P * jj * k]);
         }
         bfly<C2, false>(y[b]);
@@ -352,7 +378,7 @@ __device__ __forceinline__ void plane_load_cols(cpx<T>* x, cpx<T>& xn, const cpx
                                                 __amdgpu_buffer_rsrc_t rn, int item, int tid) {
   constexpr int TPC = Q / C1;
   const int plane = item / P, p = item - (item / P) * P;
-  const int cs = tid % M, l = tid / M;
+  const int cs = col_cs<TPC>(tid), l = col_l<TPC>(tid);
   const cpx<T>* src = ws + ((int64_t)plane * (P * Q) + p * Q) * M;
 #pragma unroll
   for (int kk = 0; kk < C1; kk++) x[kk] = src[(l + kk * TPC) * M + cs];
@@ -375,6 +401,8 @@ __global__ void __launch_bounds__(Q * M / R1)
   static_assert(R1 == 16 && 16 % TPR == 0 && M % 16 == 0,
                 "LDS offsets split into lane + constant parts");
   static_assert(64 % TPR == 0, "the threads of a row lie in one wave");
+  static_assert(64 % TPC == 0 && M % (64 / TPC) == 0 && NT % 64 == 0,
+                "the threads of a column lie in one wave");
   extern __shared__ __align__(16) unsigned char smem[];
   cpx<T>* buf = reinterpret_cast<cpx<T>*>(smem);  // rows of M entries (padded)
   cpx<T>* nyl = buf + prow_size(Q, M);             // X[q][M] of every row
@@ -411,7 +439,7 @@ __global__ void __launch_bounds__(Q * M / R1)
         __syncthreads();
       }
     }
-    const int cs = tl % M, l = tl / M;
+    const int cs = col_cs<TPC>(tl), l = col_l<TPC>(tl);
     const int q = tl / TPR, j = tl - (tl / TPR) * TPR;
     const int plane = item / P;
     const int p = item - plane * P;
@@ -428,29 +456,40 @@ __global__ void __launch_bounds__(Q * M / R1)
     }
     bfly<C1, true>(x);
     {
-      // column exchange: element idx of column cs at row idx (stride M)
-      cpx<T>* cp = buf + prow<M>(l * C1) + pcol(cs);
+      // column exchange: element m of sub-sequence l of column cs at row
+      // l C1 + (m ^ (l & 1))
+      const int lb = l & 1;
+      cpx<T>* ce = buf + prow<M>(l * C1 + lb) + pcol(cs);
+      cpx<T>* co = buf + prow<M>(l * C1 - lb) + pcol(cs);
 #pragma unroll
-      for (int kk = 0; kk < C1; kk++) cp[prow<M>(kk)] = x[kk];
+      for (int kk = 0; kk < C1; kk++) ((kk & 1) ? co : ce)[prow<M>(kk)] = x[kk];
     }
     // prefetch the next work item
     if (PF && item + (int)gridDim.x < nitems)
       plane_load_cols<T, M, C1, Q, P, PACK>(x, xn, ws, rn, item + gridDim.x, tl);
-    __syncthreads();
+    // the column's TPC threads are lanes of one wave
+    wave_lds_sync();
     cpx<T> y[C1 / C2][C2];
     cpx<T>* cp = buf + prow<M>(l) + pcol(cs);
+    {
+      const cpx<T>* co = buf + prow<M>(l ^ 1) + pcol(cs);
 #pragma unroll
-    for (int b = 0; b < C1 / C2; b++) {
-      const int jj = l + b * TPC;
+      for (int b = 0; b < C1 / C2; b++) {
+        const int jj = l + b * TPC;
 #pragma unroll
-      for (int k = 0; k < C2; k++) {
-        y[b][k] = cp[prow<M>(b * TPC + k * C1)];
-        if (k) y[b][k] = cmulc(y[b][k], twc[P * jj * k]);
+        for (int k = 0; k < C2; k++) {
+          y[b][k] = ((k & 1) ? co : (const cpx<T>*)cp)[prow<M>(b * TPC + k * C1)];
+          if (k) y[b][k] = cmulc(y[b][k], twc[P * jj * k]);
+        }
+        bfly<C2, true>(y[b]);
       }
-      bfly<C2, true>(y[b]);
     }
-    // rows q = jj + C1 k, natural order, written back in place (a thread
-    // rewrites exactly the entries it read: no barrier); unpack columns 0 and M
+    // the write-back below goes to natural rows, which the wave's other
+    // threads read above (swizzled)
+    wave_lds_sync();
+    // rows q = jj + C1 k, natural order, written back into the column (its
+    // entries are read and written by the lanes of one wave); unpack columns
+    // 0 and M
 #pragma unroll
     for (int b = 0; b < C1 / C2; b++)
 #pragma unroll
