@@ -25,6 +25,7 @@ struct bri17_batch {
   int spin_limit = 1 << 24;
   int last_nitems = 0, last_grid = 0, last_threads = 0, last_dtype = 0;
   size_t last_lds = 0, last_off_items = 0;
+  int graphs = 0;  // live graphs captured from this context (they replay its tables)
   std::mutex mtx;
 };
 
@@ -77,6 +78,10 @@ int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* 
   if (!b || !items || n < 1) return b17_err(BRI17_ERR_INVALID, "invalid argument");
   if (n > 65535) return b17_err(BRI17_ERR_INVALID, "at most 65535 instances per batch");
   std::lock_guard<std::mutex> g(b->mtx);
+  if (b->graphs)
+    return b17_err(BRI17_ERR_STATE,
+                   "this batch context is captured in a graph, which replays its device tables: "
+                   "run other batches on another context, or destroy the graph first");
   const int dtype = items[0].op ? items[0].op->plan->dtype : -1;
   std::set<const bri17_operator*> seen;
   for (int i = 0; i < n; i++) {
@@ -194,6 +199,24 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
 }
 
 int* b17_batch_ctl(bri17_batch* b) { return b ? b->d_ctl : nullptr; }
+
+// A graph replays the context's device tables and every member operator's
+// workspace: while it lives, the context runs nothing else (bri17_batch_run
+// refuses) and the workspaces are pinned (never reallocated).
+int b17_batch_pin(bri17_batch* b, const bri17_batch_item* items, int n) {
+  std::lock_guard<std::mutex> g(b->mtx);
+  for (int i = 0; i < n; i++) {
+    std::lock_guard<std::mutex> og(items[i].op->mtx);
+    items[i].op->ws_pinned = true;
+  }
+  b->graphs++;
+  return BRI17_OK;
+}
+
+void b17_batch_unpin(bri17_batch* b) {
+  std::lock_guard<std::mutex> g(b->mtx);
+  if (b->graphs > 0) b->graphs--;
+}
 
 // Launch the last batch again from the tables already on the device
 // (counter reset + persistent kernel only), e.g. inside a graph capture.

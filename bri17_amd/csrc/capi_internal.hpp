@@ -54,6 +54,10 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
                      int threads, size_t lds, hipStream_t st);
 // device address of the control words [head, error, done...] of the last launch
 int* b17_batch_ctl(bri17_batch* b);
+// a graph of the context's last launch is created / destroyed: pins the
+// member operators' workspaces, and refuses bri17_batch_run while any lives
+int b17_batch_pin(bri17_batch* b, const bri17_batch_item* items, int n);
+void b17_batch_unpin(bri17_batch* b);
 
 namespace b17 {
 // tuning registered with bri17_tuning_cache_set for a grid, if any

@@ -19,6 +19,7 @@
 struct bri17_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  bri17_batch* batch = nullptr;  // batch graphs: the context whose tables they replay
 };
 
 namespace {
@@ -118,7 +119,16 @@ int bri17_batch_capture(bri17_batch* b, const bri17_batch_item* items, int n, br
   int to = 0;
   rc = bri17_batch_wait(b, &to);
   if (rc) return rc;
-  return capture([&](hipStream_t st) { return b17_batch_relaunch(b, st); }, graph);
+  // the graph holds the context's tables and every member's workspace
+  rc = b17_batch_pin(b, items, n);
+  if (rc) return rc;
+  rc = capture([&](hipStream_t st) { return b17_batch_relaunch(b, st); }, graph);
+  if (rc) {
+    b17_batch_unpin(b);
+    return rc;
+  }
+  (*graph)->batch = b;
+  return BRI17_OK;
 }
 
 int bri17_graph_launch(bri17_graph* g, void* stream) {
@@ -132,6 +142,7 @@ int bri17_graph_destroy(bri17_graph* g) {
   if (!g) return BRI17_OK;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->batch) b17_batch_unpin(g->batch);
   delete g;
   return BRI17_OK;
 }

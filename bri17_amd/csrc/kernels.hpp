@@ -96,6 +96,8 @@ struct PencilParams {
   int mode;     // 0: K u, 1: K^-1 f, 2: strain B u, 3: eigenstress -> -strain
   int xcd_remap;
   double mu, s;   // material constants with the apply / solve scale folded in
+                  // (modes 1 and 3: also divided by the block normalisation)
+  double kinv;    // modes 1 and 3: factor of the normalised block's inverse
   double scale;   // |h|/|N| (apply) or 1/(|h||N|) (solve), for reference
   double bscale;  // modes 2 and 3: factor of the strain output (1/|N|, 2/|N|)
   FastDiv wdiv, pwdiv, rsdiv;

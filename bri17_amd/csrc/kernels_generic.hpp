@@ -364,7 +364,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
   }
 
   {
-    const T mu = T(pp.mu), sc = T(pp.s);
+    const T mu = T(pp.mu), sc = T(pp.s), kinv = T(pp.kinv);
     const int items = N0 * PW;
     for (int e = tid; e < items; e += nthr) {
       const int k0 = (int)fdiv((uint32_t)e, pp.pwdiv);
@@ -381,7 +381,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
 #pragma unroll
       for (int c = 0; c < CIN; c++) x[c] = lds[c * cst + e];
       if constexpr (MODE <= 1) {
-        modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, x);
+        modal_block<T, DIM, MODE>(coef, k0, k1, kl, mu, sc, x, kinv);
       } else {
         const gptr<const StrainCoef<T>> scf[3] = {(gptr<const StrainCoef<T>>)pp.scoef[0],
                                                   (gptr<const StrainCoef<T>>)pp.scoef[1],
@@ -389,7 +389,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
         if constexpr (MODE == 2)
           strain_block<T, DIM>(scf, k0, k1, kl, T(pp.bscale), x);
         else
-          eigenstress_block<T, DIM>(coef, scf, k0, k1, kl, mu, sc, T(pp.bscale), x);
+          eigenstress_block<T, DIM>(coef, scf, k0, k1, kl, mu, sc, T(pp.bscale), kinv, x);
       }
 #pragma unroll
       for (int c = 0; c < COUT; c++) lds[c * cst + e] = x[c];

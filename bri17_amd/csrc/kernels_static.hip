@@ -733,7 +733,7 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
       s_freq(pp, st.nyq, f0 + bw, ka, kl);
       if (bw < wv) {
         const int k1 = ka + pp.q * bkb;
-        const T mu = T(pp.mu), s = T(pp.s);
+        const T mu = T(pp.mu), s = T(pp.s), kinv = T(pp.kinv);
         const ModalCoef<T> b = coef1[k1], cc = coef2[kl];
         const T chichi = b.chi * cc.chi, phichi = b.phi * cc.chi, chiphi = b.chi * cc.phi;
         const T s_psichi = s * b.psi * cc.chi, s_chipsi = s * b.chi * cc.psi;
@@ -747,7 +747,7 @@ __global__ void __launch_bounds__(3 * W * (N0 / A0))
 #pragma unroll
           for (int q3 = 0; q3 < 3; q3++) xx[q3] = buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)];
           sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
-                              a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
+                              a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx, kinv);
 #pragma unroll
           for (int q3 = 0; q3 < 3; q3++) buf[pd(((q3 * P + bkb) * N0 + k0) * W + bw)] = xx[q3];
         }
@@ -1022,7 +1022,7 @@ __global__ void __launch_bounds__(W * R * P)
     pencil_r_fft<T, R, W, false>(X[c], buf + (c & 1) * BS, tw, kb * R, l, w);  // (its barrier also covers tw)
   }
 
-  const T mu = T(pp.mu), s = T(pp.s);
+  const T mu = T(pp.mu), s = T(pp.s), kinv = T(pp.kinv);
   cpx<T>* E = buf + BS;  // free until the first inverse transform
   if (t == 0) {
     // row 0, k = 0: the spectra of the packed real pair, before the block
@@ -1051,7 +1051,7 @@ __global__ void __launch_bounds__(W * R * P)
       const T dg = mu * (H0 + H1 + H2);
       cpx<T> xx[3] = {X[0][j], X[1][j], X[2][j]};
       sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * s_psichi,
-                          a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx);
+                          a.psi * s_chipsi, a.chi * s_psipsi, kz && k0 == 0, xx, kinv);
       X[0][j] = xx[0];
       X[1][j] = xx[1];
       X[2][j] = xx[2];
@@ -1068,7 +1068,8 @@ __global__ void __launch_bounds__(W * R * P)
         const T H2 = a.chi * (b.chi * cc.phi);
         const T dg = mu * (H0 + H1 + H2);
         sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, a.psi * (s * b.psi * cc.chi),
-                            a.psi * (s * b.chi * cc.psi), a.chi * (s * b.psi * cc.psi), zero, xx);
+                            a.psi * (s * b.chi * cc.psi), a.chi * (s * b.psi * cc.psi), zero, xx,
+                            kinv);
       };
 #pragma unroll
       for (int j = 0; j < R; j++) {

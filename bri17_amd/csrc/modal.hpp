@@ -5,8 +5,13 @@
 // The scalar factor of the apply (|h|/|N|) or of the Green solve
 // (1/(|h||N|)) is folded into the material constants by the host:
 //   apply: mu' = mu c, s' = s c          ->  y = K'(mu', s') x = c K x
-//   solve: mu' = mu / c, s' = s / c      ->  y = K'^-1 x = c K^-1 x
-// with s = mu / (1 - 2 nu) = lambda + mu.
+//   solve: mu' = mu / (c k), s' = s / (c k), kinv = 1 / k
+//                                        ->  y = kinv K'^-1 x = c K^-1 x
+// with s = mu / (1 - 2 nu) = lambda + mu.  The normalisation of the
+// inverted modes, k = (mu / c + s / c) 4 / h_min^2, brings the block's
+// entries to O(1) whatever the units of mu and L: its determinant (cubic in
+// the entries) would otherwise leave the fp32 range for mu above about 1e7 on
+// a 256^3 unit cube.  The eigenstress mode (3) is normalised the same way.
 #pragma once
 
 #include "fft_core.hpp"
@@ -30,11 +35,11 @@ __device__ __forceinline__ T block_recip(T d) {
   }
 }
 
-// y = K x (MODE 0) or y = K^-1 x (MODE 1, y = 0 when `null`) for the real
-// symmetric 3x3 block K = [[K00,K01,K02],[K01,K11,K12],[K02,K12,K22]].
+// y = K x (MODE 0) or y = kinv K^-1 x (MODE 1, y = 0 when `null`) for the
+// real symmetric 3x3 block K = [[K00,K01,K02],[K01,K11,K12],[K02,K12,K22]].
 template <typename T, int MODE>
 __device__ __forceinline__ void sym3_apply(T K00, T K11, T K22, T K01, T K02, T K12, bool null,
-                                           cpx<T>* x) {
+                                           cpx<T>* x, T kinv = T(1)) {
   if constexpr (MODE == 0) {
     const cpx<T> y0 = x[0] * K00 + x[1] * K01 + x[2] * K02;
     const cpx<T> y1 = x[0] * K01 + x[1] * K11 + x[2] * K12;
@@ -54,7 +59,7 @@ __device__ __forceinline__ void sym3_apply(T K00, T K11, T K22, T K01, T K02, T 
     const T A11 = K00 * K22 - K02 * K02;
     const T A12 = K01 * K02 - K00 * K12;
     const T A22 = K00 * K11 - K01 * K01;
-    const T r = block_recip(K00 * A00 + K01 * A01 + K02 * A02);
+    const T r = block_recip(K00 * A00 + K01 * A01 + K02 * A02) * kinv;
     const cpx<T> y0 = (x[0] * A00 + x[1] * A01 + x[2] * A02) * r;
     const cpx<T> y1 = (x[0] * A01 + x[1] * A11 + x[2] * A12) * r;
     const cpx<T> y2 = (x[0] * A02 + x[1] * A12 + x[2] * A22) * r;
@@ -65,7 +70,8 @@ __device__ __forceinline__ void sym3_apply(T K00, T K11, T K22, T K01, T K02, T 
 }
 
 template <typename T, int MODE>
-__device__ __forceinline__ void sym2_apply(T K00, T K11, T K01, bool null, cpx<T>* x) {
+__device__ __forceinline__ void sym2_apply(T K00, T K11, T K01, bool null, cpx<T>* x,
+                                           T kinv = T(1)) {
   if constexpr (MODE == 0) {
     const cpx<T> y0 = x[0] * K00 + x[1] * K01;
     const cpx<T> y1 = x[0] * K01 + x[1] * K11;
@@ -76,7 +82,7 @@ __device__ __forceinline__ void sym2_apply(T K00, T K11, T K01, bool null, cpx<T
       x[0] = x[1] = cpx<T>{T(0), T(0)};
       return;
     }
-    const T r = block_recip(K00 * K11 - K01 * K01);
+    const T r = block_recip(K00 * K11 - K01 * K01) * kinv;
     const cpx<T> y0 = (x[0] * K11 - x[1] * K01) * r;
     const cpx<T> y1 = (x[1] * K00 - x[0] * K01) * r;
     x[0] = y0;
@@ -89,12 +95,12 @@ __device__ __forceinline__ void sym2_apply(T K00, T K11, T K01, bool null, cpx<T
 //   K_ij = s psi_i psi_j prod_{m not in {i,j}} chi_m   (2D: no chi product).
 template <typename T, int DIM, int MODE, typename CP>
 __device__ __forceinline__ void modal_block(const CP* coef, int k0, int k1,
-                                            int kl, T mu, T s, cpx<T>* x) {
+                                            int kl, T mu, T s, cpx<T>* x, T kinv = T(1)) {
   if constexpr (DIM == 2) {
     const ModalCoef<T> a = coef[0][k0], b = coef[1][kl];
     const T H0 = a.phi * b.chi, H1 = a.chi * b.phi;
     const T dg = mu * (H0 + H1);
-    sym2_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * a.psi * b.psi, k0 == 0 && kl == 0, x);
+    sym2_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * a.psi * b.psi, k0 == 0 && kl == 0, x, kinv);
   } else {
     const ModalCoef<T> a = coef[0][k0], b = coef[1][k1], c = coef[2][kl];
     const T H0 = a.phi * b.chi * c.chi;
@@ -103,7 +109,7 @@ __device__ __forceinline__ void modal_block(const CP* coef, int k0, int k1,
     const T dg = mu * (H0 + H1 + H2);
     sym3_apply<T, MODE>(s * H0 + dg, s * H1 + dg, s * H2 + dg, s * a.psi * b.psi * c.chi,
                         s * a.psi * b.chi * c.psi, s * a.chi * b.psi * c.psi,
-                        k0 == 0 && k1 == 0 && kl == 0, x);
+                        k0 == 0 && k1 == 0 && kl == 0, x, kinv);
   }
 }
 
@@ -158,9 +164,10 @@ __device__ __forceinline__ void strain_block(const SP* sc, int k0, int k1, int k
 // Mode 3: bri17.hpp:308-355 at k, times bscale.  rhs = tau_mat conj(B^),
 // u = K^-1 rhs, eta = sym(B^ u); with B^ = 2i E beta and |E| = 1:
 // eta_ab = 2 (beta_a v_b + beta_b v_a), v = K^-1 (tau_mat beta).  Zero at k = 0.
+// The host passes the normalised block (mu / k, s / k) and kinv = 1 / k.
 template <typename T, int DIM, typename CP, typename SP>
 __device__ __forceinline__ void eigenstress_block(const CP* coef, const SP* sc, int k0, int k1,
-                                                  int kl, T mu, T s, T bscale, cpx<T>* x) {
+                                                  int kl, T mu, T s, T bscale, T kinv, cpx<T>* x) {
   constexpr T r2 = T(1.4142135623730950488016887242096981L);
   constexpr T ir2 = T(0.70710678118654752440084436210484903L);
   constexpr int S = Voigt<DIM>::n;
@@ -184,7 +191,7 @@ __device__ __forceinline__ void eigenstress_block(const CP* coef, const SP* sc, 
     for (int b = 1; b < DIM; b++) acc = acc + t[a][b] * beta[b];
     v[a] = acc;
   }
-  modal_block<T, DIM, 1>(coef, k0, k1, kl, mu, s, v);
+  modal_block<T, DIM, 1>(coef, k0, k1, kl, mu, s, v, kinv);
   if (null) {
 #pragma unroll
     for (int i = 0; i < S; i++) x[i] = cpx<T>{T(0), T(0)};

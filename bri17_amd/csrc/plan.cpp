@@ -409,6 +409,7 @@ PencilParams pencil_for(const Plan& pl, double mu, double nu, int mode) {
   const double s = mu / (1.0 - 2.0 * nu);
   pc.mu = mode == 0 ? mu * pc.scale : mu / pc.scale;
   pc.s = mode == 0 ? s * pc.scale : s / pc.scale;
+  pc.kinv = 1.0;
   if (mode >= 2) {
     // strain / eigenstress: the per-k map of the reference (K^ itself,
     // bri17.hpp:247-292), inverse DFT divided by |N| (tests/test_bri17.cpp:230-237)
@@ -416,6 +417,25 @@ PencilParams pencil_for(const Plan& pl, double mu, double nu, int mode) {
     pc.mu = mu;
     pc.s = s;
     pc.bscale = 1.0 / N;
+  }
+  if (mode == 1 || mode == 3) {
+    // The inverted block is built normalised: every entry of K(mu, s) is at
+    // most (s + DIM mu) max_d phi_d <= (s + DIM mu) 4 / h_min^2, so dividing
+    // mu and s by k = (s + mu) 4 / h_min^2 leaves entries of order one and a
+    // determinant well inside the fp32 range whatever the units of mu and L.
+    // The inverse is then scaled back by kinv = 1 / k (one product per
+    // frequency, on the reciprocal of the determinant).
+    double hmin2 = 0.0;
+    for (int d = 0; d < pl.dim; d++) {
+      const double h = pl.L[d] / pl.shape[d];
+      hmin2 = d == 0 ? h * h : std::min(hmin2, h * h);
+    }
+    const double k = (std::fabs(pc.mu) + std::fabs(pc.s)) * 4.0 / hmin2;
+    if (k > 0.0 && std::isfinite(k)) {
+      pc.mu /= k;
+      pc.s /= k;
+      pc.kinv = 1.0 / k;
+    }
   }
   return pc;
 }

@@ -18,8 +18,14 @@ import numpy as np
 
 from . import _lib
 
-__author__ = "bri17_amd (interface of S. Brisard's bri17)"
+# Module attributes of pybri17 (python/pybri17.cpp:100-101), set there from
+# the project metadata (metadata/author.txt, metadata/version.txt) that the
+# reference's builds pass in (python/setup.py:36-39, python/CMakeLists.txt:4-5):
+# they describe the interface, which callers may check.  The implementation
+# behind it is bri17_amd (``__implementation__``).
+__author__ = "S. Brisard"
 __version__ = "0.1"
+__implementation__ = "bri17_amd (MI355X / gfx950)"
 
 _TYPEID = {np.float64: "d", np.float32: "f"}
 

@@ -132,7 +132,14 @@ int bri17_operator_info(const bri17_operator* op, int64_t* num_dofs,
  * (tests/test_bri17.cpp:56-107) -- DFT, K^(k) u^(k), inverse DFT, times
  * |h|/|N| -- on device pointers, asynchronously on `stream` (a hipStream_t,
  * NULL = default stream).  u and Ku are DIM*|N| values of the operator's
- * dtype; they may not alias. */
+ * dtype; they may not alias.
+ *
+ * Ordering: an operator owns one half-spectrum workspace, which every device
+ * call of that operator uses (apply, solve, strain, eigenstress, its graphs,
+ * and batches it is a member of).  Calls of one operator must therefore be
+ * ordered on the device -- issued on one stream, or on streams the caller
+ * orders with events.  Two operators of the same grid (sharing tables) run
+ * concurrently without restriction. */
 int bri17_operator_apply(bri17_operator* op, const void* u, void* Ku, void* stream);
 
 /* Green solve K u = f with the reference conventions (no reference entry
@@ -200,7 +207,12 @@ int bri17_batch_info(const bri17_batch* b, int* work_items, int* workgroups, int
 /* A captured HIP graph of one operator call on fixed device buffers, or of
  * one batch, replayed with a single launch (no reference counterpart).  The
  * operator, batch context and buffers must outlive the graph; replays reuse
- * the operator's workspace, so replays of one graph must not overlap.
+ * the operator's workspace, so replays of one graph must not overlap.  While
+ * a graph lives, the workspaces it replays are pinned (a call that would grow
+ * one -- a first strain / eigenstress call -- fails with BRI17_ERR_STATE), and
+ * a batch context it was captured from refuses bri17_batch_run and
+ * bri17_batch_capture with BRI17_ERR_STATE (its device tables are the
+ * graph's); bri17_graph_destroy releases both.
  * mode: 0 apply, 1 solve, 2 strain, 3 eigenstress. */
 typedef struct bri17_graph bri17_graph;
 /* Runs the call once eagerly, then captures it. */

@@ -58,6 +58,44 @@ def test_batch_graph_replays_match_eager(device):
             assert torch.equal(o, e), "replay %d instance %d differs" % (r, i)
 
 
+def test_batch_graph_pins_context_and_workspaces(device):
+    """A batch graph replays its context's device tables and its members'
+    workspaces: while it lives, another run on that context is refused
+    (BRI17_ERR_STATE), and so is a strain call that would grow a member's
+    workspace; replays stay correct after the refusals, and the context runs
+    again once the graph is destroyed."""
+    import torch
+    from bri17_amd import Batch, StiffnessMatrix, _lib
+    spec = [((32, 48, 64), "apply"), ((96, 64), "solve"), ((48, 48, 32), "solve")]
+    ops = [StiffnessMatrix(s, None, 0.8 + 0.1 * i, 0.2, "float32") for i, (s, _) in enumerate(spec)]
+    g = torch.Generator(device=device).manual_seed(9)
+    xs = [torch.randn(op.field_shape, generator=g, device=device) for op in ops]
+    eager = [(op.apply(x) if m == "apply" else op.solve(x)) for op, (s, m), x in zip(ops, spec, xs)]
+    outs = [torch.empty_like(x) for x in xs]
+    items = [(op, m, x, o) for op, (s, m), x, o in zip(ops, spec, xs, outs)]
+    b = Batch()
+    graph = b.capture(items)
+    with pytest.raises(_lib.Bri17Error) as e:
+        b.run(items)
+    assert e.value.code == _lib.BRI17_ERR_STATE
+    with pytest.raises(_lib.Bri17Error) as e:
+        ops[0].strain(xs[0])
+    assert e.value.code == _lib.BRI17_ERR_STATE
+    for r in range(2):
+        for o in outs:
+            o.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.launch()
+        torch.cuda.synchronize()
+        for i, (o, ex) in enumerate(zip(outs, eager)):
+            assert torch.equal(o, ex), "replay %d instance %d differs" % (r, i)
+    graph.close()
+    b.run(items)
+    assert b.wait() == 0
+    for o, ex in zip(outs, eager):
+        assert torch.equal(o, ex)
+
+
 def test_autotune_registers_and_roundtrips_json(device, oracle, tmp_path):
     import torch
     from bri17_amd import StiffnessMatrix, autotune

@@ -19,6 +19,8 @@ pytestmark = pytest.mark.gpu
 NT = min(16, os.cpu_count() or 1)
 # cubic sides 2^a 3^b within [32, 128] and a few 2D grids
 SIDES = (32, 36, 48, 54, 64, 72, 81, 96, 108, 128)
+# every side of the serving workload: log-uniform in [32, 256] rounded to 2^a 3^b
+SIDES_SERVE = (32, 36, 48, 54, 64, 72, 81, 96, 108, 128, 144, 162, 192, 216, 243, 256)
 
 
 def _spec(seed=5, n=28):
@@ -62,10 +64,29 @@ def _serve(server, spec, xs, burst=True):
     return [r.out for r in reqs], res
 
 
-def test_alone_vs_mix_bitwise(device, oracle):
+def _spec_serving(seed=8, n=24):
+    """The serving workload's mix: fp32 cubes with side log-uniform in
+    [32, 256] rounded (in log space) to 2^a 3^b, 70% K u / 30% Green solve,
+    mu ~ U[0.5, 2], nu ~ U[0, 0.45]; every side of the set appears at least
+    once, the large runtime-size sides 144..243 and the fixed-size 256 among
+    them."""
+    rng = np.random.default_rng(seed)
+    logs = np.log(np.array(SIDES_SERVE, dtype=float))
+    sides = list(SIDES_SERVE)
+    while len(sides) < n:
+        v = rng.uniform(np.log(32), np.log(256))
+        sides.append(SIDES_SERVE[int(np.argmin(np.abs(logs - v)))])
+    rng.shuffle(sides)
+    return [dict(shape=(int(n_),) * 3, dtype="float32", mode="solve" if rng.uniform() < 0.3 else "apply",
+                 L=(1.0, 1.0, 1.0), mu=float(rng.uniform(0.5, 2.0)), nu=float(rng.uniform(0.0, 0.45)))
+            for n_ in sides]
+
+
+@pytest.mark.parametrize("mix", ["mixed_grids", "serving_sides"])
+def test_alone_vs_mix_bitwise(device, oracle, mix):
     import torch
     from bri17_amd import Server, StiffnessMatrix
-    spec = _spec()
+    spec = _spec() if mix == "mixed_grids" else _spec_serving()
     xs = _inputs(device, spec)
     print("requests:", [(s["shape"], s["dtype"][-2:], s["mode"]) for s in spec], flush=True)
     # alone: one request at a time on an idle server
