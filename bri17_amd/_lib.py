@@ -50,6 +50,7 @@ SIGNATURES = {
     "bri17_operator_eigenstress": (_c.c_int, [_vp, _vp, _vp, _vp]),
     "bri17_operator_strain_host": (_c.c_int, [_vp, _vp, _vp]),
     "bri17_operator_eigenstress_host": (_c.c_int, [_vp, _vp, _vp]),
+    "bri17_operator_run_noalloc": (_c.c_int, [_vp, _c.c_int, _vp, _vp, _vp]),
     "bri17_operator_create_tuned": (
         _c.c_int, [_c.c_int, _ip, _dp, _c.c_double, _c.c_double, _c.c_int, _vp, _c.POINTER(_vp)]),
     "bri17_operator_describe": (_c.c_int, [_vp, _c.c_char_p, _c.c_size_t]),

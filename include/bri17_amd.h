@@ -174,6 +174,27 @@ int bri17_operator_eigenstress(bri17_operator* op, const void* tau, void* eta, v
 int bri17_operator_strain_host(bri17_operator* op, const void* u, void* eps);
 int bri17_operator_eigenstress_host(bri17_operator* op, const void* tau, void* eta);
 
+/* -------------------------------------------------- no-allocation call */
+
+/* No counterpart in the reference: an entry point for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The operator call of `mode` -- 0 K u,
+ * 1 Green solve, 2 strain, 3 eigenstress, exactly as bri17_operator_apply /
+ * _solve / _strain / _eigenstress, on device pointers, asynchronously on
+ * `stream` -- with its half-spectrum workspace taken from a static device
+ * array of BRI17_NOALLOC_SCRATCH_BYTES that the library holds on every
+ * device it runs this on, until the process ends.  It allocates nothing (the
+ * operator, created beforehand, holds the tables) and leaves the operator's
+ * own workspace untouched.  A grid and mode whose workspace exceeds the
+ * array (e.g. the 201 MB of a 256^3 fp32 K u; 128^3 fits in both dtypes) is
+ * refused with BRI17_ERR_NOMEM before anything is launched.  All such calls
+ * on one device share that device's array: calls that overlap on one device
+ * (on different streams, or with a graph replay of such a call) race and
+ * give wrong results, so order them. */
+#define BRI17_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int bri17_operator_run_noalloc(bri17_operator* op, int mode, const void* in, void* out,
+                               void* stream);
+
 /* ---------------------------------------------------------------- batches */
 
 /* Many operators -- different grids, dimensions and materials, one dtype --
