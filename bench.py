@@ -44,6 +44,25 @@ def _kernel_bytes(shape, esz):
     return {"plane_r2c": field + half, "pencil_block": 2 * half, "plane_c2r": half + field}
 
 
+def _settle(fn, sync, seconds):
+    """Run `fn` back to back for `seconds` (at least 20 calls).  From idle,
+    the device's power management runs the first ~100 applies of this
+    workload at varying clocks (measured per step with events: ~255 us for
+    the first few, ~300 us over steps 5-25, ~246 us from step ~100 on), so a
+    short timed window measures that transient, not the kernels.  Returns
+    the number of calls made."""
+    sync()
+    t0 = time.perf_counter()
+    i = 0
+    while i < 20 or time.perf_counter() - t0 < seconds:
+        fn(i)
+        i += 1
+        if i % 20 == 0:
+            sync()
+    sync()
+    return i
+
+
 def _timed(fn, steps, sync):
     sync()
     t0 = time.perf_counter()
@@ -114,8 +133,10 @@ def cpu_baseline(f_host, shape, mu, nu, threads, gpu_apply=None, gpu_solve=None)
     recomputation, and of the Green solve with the per-k inverse and the
     k = 0 branch of bri17.hpp:336-341) timed on this host, on one 256^3 apply
     and one solve each (the faster of two runs), plus one single-thread
-    apply.  Its results are also the checker of the GPU's fp32 results on
-    the same (fp32-rounded) input."""
+    apply, the 64-instance batched workload one instance after the other,
+    and a 24-request sample of the serving mix one request after the other.
+    Its results are also the checker of the GPU's fp32 results on the same
+    (fp32-rounded) input."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     pyoracle.lib()
@@ -141,6 +162,33 @@ def cpu_baseline(f_host, shape, mu, nu, threads, gpu_apply=None, gpu_solve=None)
         t0 = time.perf_counter()
         pyoracle.apply(shape, L, mu, nu, u, nthreads=1)
         out["single_thread_applies_per_s"] = 1.0 / (time.perf_counter() - t0)
+    # the 64-instance batched workload, one instance after the other
+    import numpy as np
+    t0 = time.perf_counter()
+    nodes = 0
+    for i, (sh, bmu, bnu, m) in enumerate(batch_spec()):
+        x = pyoracle.fill_uniform(900 + i, len(sh) * int(np.prod(sh))).reshape((len(sh),) + sh)
+        (pyoracle.apply if m == "apply" else pyoracle.solve)(sh, (1.0,) * len(sh), bmu, bnu, x,
+                                                            nthreads=threads)
+        nodes += int(np.prod(sh))
+    t_b = time.perf_counter() - t0
+    out["batched_64"] = {"instances_per_s": 64 / t_b, "ms_per_batch": 1e3 * t_b, "nodes": nodes}
+    # a sample of the serving mix, served one request after the other
+    rng = np.random.default_rng(55)
+    sample = serving_draw(rng, 24)
+    lat = []
+    for i, (side, m, smu, snu) in enumerate(sample):
+        sh = (int(side),) * 3
+        x = pyoracle.fill_uniform(950 + i, 3 * side ** 3).reshape((3,) + sh)
+        t0 = time.perf_counter()
+        (pyoracle.apply if m == "apply" else pyoracle.solve)(sh, (1.0, 1.0, 1.0), float(smu), float(snu),
+                                                            x, nthreads=threads)
+        lat.append(time.perf_counter() - t0)
+    out["serving_sample"] = {"requests": len(sample), "requests_per_s": len(sample) / sum(lat),
+                             "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                             "p99_ms": 1e3 * float(np.percentile(lat, 99)),
+                             "note": "compute time of each request alone (no queueing), same mix "
+                                     "as the served workload"}
     errs = {}
     if gpu_apply is not None:
         errs["apply"] = _errs(gpu_apply, ref_apply)
@@ -199,6 +247,30 @@ SIDES2_BATCH = (64, 72, 81, 96, 108, 128, 144, 162, 192, 216, 243, 256, 288, 324
 SIDES_SERVE = (32, 36, 48, 54, 64, 72, 81, 96, 108, 128, 144, 162, 192, 216, 243, 256)
 
 
+def batch_spec(seed=4):
+    """(shape, mu, nu, mode) of the 64 instances of the batched workload."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    spec = []
+    for i in range(64):
+        shape = tuple(int(rng.choice(SIDES2_BATCH if i < 16 else SIDES3_BATCH)) for _ in range(2 if i < 16 else 3))
+        spec.append((shape, float(rng.uniform(0.5, 2.0)), float(rng.uniform(0.0, 0.45)),
+                     "solve" if rng.uniform() < 0.3 else "apply"))
+    return spec
+
+
+def serving_draw(rng, n):
+    """n requests of the serving mix: (side, mode, mu, nu), cubic fp32 grids
+    with side log-uniform in [32, 256] rounded in log space to 2^a 3^b, 70%
+    K.u / 30% Green solve, mu ~ U[0.5, 2], nu ~ U[0, 0.45]."""
+    import numpy as np
+    logs = np.log(np.array(SIDES_SERVE, dtype=float))
+    x = rng.uniform(np.log(32), np.log(256), size=n)
+    side = [SIDES_SERVE[int(np.argmin(np.abs(logs - v)))] for v in x]
+    mode = ["solve" if u < 0.3 else "apply" for u in rng.uniform(size=n)]
+    return list(zip(side, mode, rng.uniform(0.5, 2.0, size=n), rng.uniform(0.0, 0.45, size=n)))
+
+
 def batched(dev, steps, warmup, seed=4):
     """64 instances in one persistent launch, fp32: 48 3D grids with sides
     from {32,48,64,96,128}, 16 2D grids with sides 2^a 3^b in [64,512],
@@ -206,12 +278,7 @@ def batched(dev, steps, warmup, seed=4):
     import numpy as np
     import torch
     from bri17_amd import Batch, StiffnessMatrix
-    rng = np.random.default_rng(seed)
-    spec = []
-    for i in range(64):
-        shape = tuple(int(rng.choice(SIDES2_BATCH if i < 16 else SIDES3_BATCH)) for _ in range(2 if i < 16 else 3))
-        spec.append((shape, float(rng.uniform(0.5, 2.0)), float(rng.uniform(0.0, 0.45)),
-                     "solve" if rng.uniform() < 0.3 else "apply"))
+    spec = batch_spec(seed)
     g = torch.Generator(device=dev).manual_seed(seed)
     ops = [StiffnessMatrix(sh, None, mu, nu, "float32") for sh, mu, nu, _ in spec]
     xs = [torch.rand(op.field_shape, generator=g, device=dev) * 2 - 1 for op in ops]
@@ -258,13 +325,9 @@ def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
     from bri17_amd import Server
     from bri17_amd.server import poisson_arrivals, sleep_until
     rng = np.random.default_rng(seed)
-    logs = np.log(np.array(SIDES_SERVE, dtype=float))
 
     def draw(n):
-        x = rng.uniform(np.log(32), np.log(256), size=n)
-        side = [SIDES_SERVE[int(np.argmin(np.abs(logs - v)))] for v in x]
-        mode = ["solve" if u < 0.3 else "apply" for u in rng.uniform(size=n)]
-        return list(zip(side, mode, rng.uniform(0.5, 2.0, size=n), rng.uniform(0.0, 0.45, size=n)))
+        return serving_draw(rng, n)
 
     g = torch.Generator(device=dev).manual_seed(seed)
     inputs = {n: torch.rand((3, n, n, n), generator=g, device=dev) * 2 - 1 for n in SIDES_SERVE}
@@ -341,6 +404,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-instance batched launch")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--settle", type=float, default=0.3, metavar="S",
+                    help="seconds of back-to-back applies before the warmup steps, so that the "
+                         "timed steps run at the device's steady clock (0: none)")
+    ap.add_argument("--force-generic", action="store_true",
+                    help="run the headline grid on the runtime-size kernels (profiling, A/B)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 3 plain device copies of MIB MiB each (reference kernel for "
                          "calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE counters)")
@@ -370,12 +438,14 @@ def main():
     g = torch.Generator(device=dev).manual_seed(2026)
     f = torch.randn((3,) + shape, generator=g, device=dev, dtype=torch.float32)
     f -= f.mean(dim=(1, 2, 3), keepdim=True)
-    op = StiffnessMatrix(shape, (1.0, 1.0, 1.0), mu, nu, "float32")
+    op = StiffnessMatrix(shape, (1.0, 1.0, 1.0), mu, nu, "float32",
+                         tuning=dict(force_generic=1) if args.force_generic else None)
     outs = [torch.empty_like(f), torch.empty_like(f)]
 
     def step(i):
         op.apply(f, out=outs[i & 1])
 
+    settle_calls = _settle(step, sync, args.settle) if args.settle > 0 else 0
     for i in range(W):
         step(i)
     t = _timed(step, K, sync)
@@ -434,6 +504,7 @@ def main():
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
+        "device_settle": {"seconds": args.settle, "applies": settle_calls},
         "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
