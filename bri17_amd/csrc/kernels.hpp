@@ -115,6 +115,17 @@ __host__ __device__ constexpr bool fused_split(int p) {
   return p == 1 || p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16;
 }
 
+// Entries (complex T) of the twiddle tables a work item copies into LDS after
+// its data: plane items the row table (nl) and in 3D the middle-axis table
+// (n1); pencil tiles the axis-0 table (n0) and, for a split P without a fused
+// butterfly, the middle-axis table (P q = n1).
+__host__ __device__ inline int plane_table_entries(const PlaneParams& pp) {
+  return pp.nl + (pp.has_col ? pp.n1 : 0);
+}
+__host__ __device__ inline int pencil_table_entries(const PencilParams& pc) {
+  return pc.n0 + ((pc.p > 1 && !fused_split(pc.p)) ? pc.p * pc.q : 0);
+}
+
 template <typename T>
 hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp,
                             int nblocks, int nthreads, size_t lds,

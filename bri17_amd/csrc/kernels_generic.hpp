@@ -55,14 +55,36 @@ __device__ __forceinline__ cpx<T> pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
   return {e.x - o.y, e.y + o.x};
 }
 
+// Twiddle tables of a work item, copied from global memory into LDS after
+// the item's data: the row table exp(-2 pi i j / nl), j < nl, then (3D) the
+// middle-axis table exp(-2 pi i j / n1), j < n1 (plane items; the host sizes
+// the LDS with plane_table_entries), or the axis-0 table exp(-2 pi i j / n0)
+// and, for splits run as a Stockham transform, the middle-axis table (pencil
+// tiles; pencil_table_entries).  Every FFT pass reads its twiddles from
+// there: an LDS read instead of a dependent global load whose latency each
+// pass would otherwise wait out.
+template <typename T>
+__device__ __forceinline__ cpx<T>* plane_tables(cpx<T>* lds, const PlaneParams& pp) {
+  return lds + pp.q * pp.rsl;
+}
+
+template <typename T>
+__device__ __forceinline__ void copy_tables(const PlaneParams& pp, cpx<T>* twr, cpx<T>* twc,
+                                            int tid, int nthr) {
+  const gptr<const cpx<T>> gr = (gptr<const cpx<T>>)pp.tw_row;
+  for (int i = tid; i < pp.nl; i += nthr) twr[i] = gr[i];
+  if (pp.has_col) {
+    const gptr<const cpx<T>> gc = (gptr<const cpx<T>>)pp.tw_col;
+    for (int i = tid; i < pp.n1; i += nthr) twc[i] = gc[i];
+  }
+}
+
 // ------------------------------------------------------------------ plane fwd
 
 template <typename T>
 __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
                                                gptr<cpx<T>> __restrict__ ws, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
-  const gptr<const cpx<T>> twr = (gptr<const cpx<T>>)pp.tw_row;
-  const gptr<const cpx<T>> twc = (gptr<const cpx<T>>)pp.tw_col;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int plane = blk / pp.p;
   const int p = blk - plane * pp.p;
@@ -73,6 +95,10 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
   }
   const int64_t row0 = (int64_t)plane * pp.n1 + p;
   const int pst = pp.p;
+  // twiddle tables in LDS after the rows (visible after the first barrier)
+  cpx<T>* const twr = plane_tables<T>(lds, pp);
+  cpx<T>* const twc = twr + pp.nl;
+  copy_tables<T>(pp, twr, twc, tid, nthr);
 
   // 1. rows -> LDS (even rows are read as m complex pairs)
   {
@@ -162,8 +188,7 @@ template <typename T>
 __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ ws,
                                                gptr<T> __restrict__ out, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
-  const gptr<const cpx<T>> twr = (gptr<const cpx<T>>)pp.tw_row;
-  const gptr<const cpx<T>> twc = (gptr<const cpx<T>>)pp.tw_col;
+  const gptr<const cpx<T>> twc_g = (gptr<const cpx<T>>)pp.tw_col;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int plane = blk / pp.p;
   const int p = blk - plane * pp.p;
@@ -174,8 +199,13 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
   }
   const int64_t row0 = (int64_t)plane * pp.n1 + p;
   const int pst = pp.p;
+  // twiddle tables in LDS after the rows (visible after the first barrier)
+  cpx<T>* const twr = plane_tables<T>(lds, pp);
+  cpx<T>* const twc = twr + pp.nl;
+  copy_tables<T>(pp, twr, twc, tid, nthr);
 
-  // 1. load (undo the split twiddle)
+  // 1. load (undo the split twiddle, read from global memory: the LDS copy
+  //    of the table is not visible before the barrier)
   {
     const int tot = nrows * pp.nlh, nlh = pp.nlh;
     gptr<const cpx<T>> src = ws + ((int64_t)plane * pp.n1 + p * pp.q) * pp.rs;
@@ -184,7 +214,7 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
       return src[(int64_t)ka * pp.rs + (e - ka * nlh)];
     }, [&](int e, cpx<T> v) {
       const int ka = (int)fdiv((uint32_t)e, pp.nlhdiv);
-      if (p > 0) v = cmulc(v, twc[p * ka]);
+      if (p > 0) v = cmulc(v, twc_g[p * ka]);
       lds[ka * pp.rsl + (e - ka * nlh)] = v;
     });
   }
@@ -305,9 +335,8 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
 // a Stockham transform per (c, i0, w) column of the tile, P elements at
 // stride W, twiddles exp(-2 pi i j / P) = tw1[j q].
 template <typename T, bool INV>
-__device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const PencilParams& pp, int nouter,
-                                              int tid, int nthr) {
-  const gptr<const cpx<T>> tw1 = (gptr<const cpx<T>>)pp.tw1;
+__device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const cpx<T>* tw1, const PencilParams& pp,
+                                              int nouter, int tid, int nthr) {
   const int W = pp.w, P = pp.p, tpt = pp.pfft.tpt, S = nthr / tpt;
   const int sl = tid % S, lane = tid / S;
   const bool inr = tid < S * tpt;
@@ -325,7 +354,7 @@ template <typename T, int DIM, int MODE>
 __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const PencilParams& pp, int t,
                                             cpx<T>* lds) {
   constexpr int CIN = ModeComps<DIM, MODE>::cin, COUT = ModeComps<DIM, MODE>::cout;
-  const gptr<const cpx<T>> tw0 = (gptr<const cpx<T>>)pp.tw0;
+  constexpr int CMAX = ModeComps<DIM, MODE>::cmax;
   const gptr<const ModalCoef<T>> coef[3] = {(gptr<const ModalCoef<T>>)pp.coef[0],
                                             (gptr<const ModalCoef<T>>)pp.coef[1],
                                             (gptr<const ModalCoef<T>>)pp.coef[2]};
@@ -334,6 +363,17 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
   const int f0 = t * W;
   const int wv = pp.f - f0 < W ? pp.f - f0 : W;
   const int cst = N0 * PW;  // LDS distance between components
+  // twiddle tables in LDS after the tile (visible after the first barrier)
+  cpx<T>* const tw0 = lds + CMAX * cst;
+  cpx<T>* const tw1 = tw0 + N0;
+  {
+    const gptr<const cpx<T>> g0 = (gptr<const cpx<T>>)pp.tw0;
+    for (int i = tid; i < N0; i += nthr) tw0[i] = g0[i];
+    if (P > 1 && !fused_split(P)) {
+      const gptr<const cpx<T>> g1 = (gptr<const cpx<T>>)pp.tw1;
+      for (int i = tid; i < P * pp.q; i += nthr) tw1[i] = g1[i];
+    }
+  }
 
   // load the tile: element (c, i0, p, w) at ((c N0 + i0) P + p) W + w
   staged_copy<kLoads<T>, cpx<T>>(CIN * N0 * PW, tid, nthr, [&](int e) {
@@ -348,7 +388,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     if (fused_split(P))
       pdft<T, false>(lds, P, W, pp.wdiv, CIN * N0, tid, nthr);
     else
-      pdft_stockham<T, false>(lds, pp, CIN * N0, tid, nthr);
+      pdft_stockham<T, false>(lds, tw1, pp, CIN * N0, tid, nthr);
     __syncthreads();  // the axis-0 passes read columns other threads wrote
   }
 
@@ -408,7 +448,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     if (fused_split(P))
       pdft<T, true>(lds, P, W, pp.wdiv, COUT * N0, tid, nthr);
     else
-      pdft_stockham<T, true>(lds, pp, COUT * N0, tid, nthr);
+      pdft_stockham<T, true>(lds, tw1, pp, COUT * N0, tid, nthr);
     __syncthreads();  // the store reads entries other threads wrote
   }
 

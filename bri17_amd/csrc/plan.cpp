@@ -18,9 +18,9 @@
 
 namespace b17 {
 
-// LDS per plane workgroup aimed for when choosing the middle-axis split: four
-// such workgroups (plus a little static LDS each) fit the 160 KiB of a CU.
-constexpr size_t kPlaneLds = 39 * 1024;
+// LDS per plane workgroup aimed for when choosing the middle-axis split
+// (rows plus twiddle tables): four such workgroups fit the 160 KiB of a CU.
+constexpr size_t kPlaneLds = 40 * 1024;
 
 static thread_local std::string t_err;
 void set_error(const std::string& msg) { t_err = msg; }
@@ -231,8 +231,11 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     // pencil tiles of at least 8 columns (64-byte rows of DIM N0 P entries);
     // any plane with such tiles; then narrower tiles, down to one column.
     // Any divisor up to 64 is a candidate; splits without a fused butterfly
-    // run a Stockham P-point transform in the pencil.
+    // run a Stockham P-point transform in the pencil.  Both kernels keep
+    // their twiddle tables in LDS after their data.
     int P = 0;
+    const size_t plane_tab = (size_t)(nl + n1) * csz;
+    auto pencil_tab = [&](int c) { return (size_t)(shape[0] + (c > 1 && !fused_split(c) ? n1 : 0)) * csz; };
     std::vector<int> cands = {1};
     for (int c = 2; c <= 64 && c <= n1; c++)
       if (n1 % c == 0 && !default_radices(c).empty()) cands.push_back(c);
@@ -242,7 +245,8 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     for (const auto& pr : prefs) {
       for (int c : cands) {
         if (tun.plane_split && c != tun.plane_split) continue;
-        if ((size_t)(n1 / c) * rsl * csz <= pr.plane && pencil_col * c * pr.min_w <= budget) {
+        if ((size_t)(n1 / c) * rsl * csz + plane_tab <= pr.plane &&
+            pencil_col * c * pr.min_w + pencil_tab(c) <= budget) {
           P = c;
           break;
         }
@@ -262,7 +266,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     nthr = std::max(nthr, round_up(std::max(pp.row.tpt, pp.col.tpt), 64));
     pl->plane_threads = nthr;
     pl->plane_blocks = dim * shape[0] * P;
-    pl->plane_lds = (size_t)Q * rsl * csz;
+    pl->plane_lds = (size_t)Q * rsl * csz + plane_tab;
     pc.p = P;
     pc.q = Q;
     pc.f = Q * nlh;
@@ -271,7 +275,9 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   } else {
     const int nrows = dim * shape[0];
     int G = std::max(1, std::min(256 / pp.row.tpt, (nrows + 255) / 256));
-    G = std::min<int>(G, (int)(budget / (rsl * csz)));
+    const size_t plane_tab = (size_t)nl * csz;
+    if (rsl * csz + plane_tab > budget) return fail(2, "a row does not fit the LDS budget for this grid");
+    G = std::min<int>(G, (int)((budget - plane_tab) / (rsl * csz)));
     G = std::max(1, std::min(G, nrows));
     pp.q = G;
     pp.p = 1;
@@ -282,7 +288,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     int nthr = std::min(1024, round_up(G * pp.row.tpt, 64));
     pl->plane_threads = nthr;
     pl->plane_blocks = (nrows + G - 1) / G;
-    pl->plane_lds = (size_t)G * rsl * csz;
+    pl->plane_lds = (size_t)G * rsl * csz + plane_tab;
     pc.p = 1;
     pc.q = 1;
     pc.f = nlh;
@@ -292,12 +298,13 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   if (!make_fft_desc(shape[0], 1, pick(tun, 2, default_radices(shape[0])), pc.ax0, &why))
     return fail(2, "axis-0 FFT: " + why);
   const size_t per_w = (size_t)dim * shape[0] * pc.p * csz;
+  const size_t ptab = (size_t)pencil_table_entries(pc) * csz;
   int W = tun.pencil_width;
   if (!W) {
     W = 1;
-    while (W < 64 && per_w * (2 * W) <= budget && 2 * W <= round_up(pc.f, 1)) W *= 2;
+    while (W < 64 && per_w * (2 * W) + ptab <= budget && 2 * W <= round_up(pc.f, 1)) W *= 2;
   }
-  if (per_w * W > budget) return fail(2, "pencil tile does not fit the LDS budget for this grid");
+  if (per_w * W + ptab > budget) return fail(2, "pencil tile does not fit the LDS budget for this grid");
   pc.w = W;
   pc.ntiles = (pc.f + W - 1) / W;
   pc.wdiv = make_fastdiv((uint32_t)W);
@@ -309,7 +316,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     nthr = std::max(nthr, round_up(pc.ax0.tpt, 64));
     if (!fused_split(pc.p)) nthr = std::max(nthr, round_up(pc.pfft.tpt, 64));
     pl->pencil_threads = nthr;
-    pl->pencil_lds = per_w * W;
+    pl->pencil_lds = per_w * W + ptab;
   }
   pl->ws_bytes = (size_t)dim * (nn / nl) * nlh * csz;
   pl->use_static = use_static;
@@ -330,8 +337,9 @@ void batch_minimum(const Plan& pl, size_t& lds, int& threads) {
   const size_t csz = 2 * pl.esz;
   const PlaneParams& pp = pl.plane;
   const PencilParams& pc = pl.pencil;
-  const size_t plane = pp.has_col ? (size_t)pp.q * pp.rsl * csz : (size_t)pp.rsl * csz;
-  const size_t pencil = (size_t)pl.dim * pc.n0 * pc.p * csz;
+  const size_t plane = (pp.has_col ? (size_t)pp.q * pp.rsl * csz : (size_t)pp.rsl * csz) +
+                       (size_t)plane_table_entries(pp) * csz;
+  const size_t pencil = (size_t)pl.dim * pc.n0 * pc.p * csz + (size_t)pencil_table_entries(pc) * csz;
   lds = std::max(plane, pencil);
   int t = std::max({pp.row.tpt, pp.has_col ? pp.col.tpt : 1, pc.ax0.tpt,
                     fused_split(pc.p) ? 1 : pc.pfft.tpt});
@@ -343,19 +351,22 @@ bool batch_geometry(const Plan& pl, size_t lds, PlaneParams& pp, int& plane_bloc
   const size_t csz = 2 * pl.esz;
   pp = pl.plane;
   plane_blocks = pl.plane_blocks;
+  const size_t plane_tab = (size_t)plane_table_entries(pp) * csz;
+  if (plane_tab >= lds) return false;
   if (!pp.has_col) {
-    int G = std::min<int>(pp.q, (int)(lds / ((size_t)pp.rsl * csz)));
+    int G = std::min<int>(pp.q, (int)((lds - plane_tab) / ((size_t)pp.rsl * csz)));
     if (G < 1) return false;
     pp.q = G;
     pp.n1 = G;
     plane_blocks = (pp.nrows + G - 1) / G;
-  } else if ((size_t)pp.q * pp.rsl * csz > lds) {
+  } else if ((size_t)pp.q * pp.rsl * csz + plane_tab > lds) {
     return false;
   }
   const size_t per_w = (size_t)pl.dim * pc.n0 * pc.p * csz;
+  const size_t ptab = (size_t)pencil_table_entries(pc) * csz;
   int W = pc.w;
-  while (W > 1 && per_w * W > lds) W /= 2;
-  if (per_w * W > lds) return false;
+  while (W > 1 && per_w * W + ptab > lds) W /= 2;
+  if (per_w * W + ptab > lds) return false;
   pc.w = W;
   pc.ntiles = (pc.f + W - 1) / W;
   pc.wdiv = make_fastdiv((uint32_t)W);
@@ -464,10 +475,11 @@ Status mode_geometry(const Plan& pl, double mu, double nu, int mode, ModeGeometr
   if (mode <= 1) return Status{};
   PencilParams& pc = g.pc;
   const size_t per_w = (size_t)cmax * pc.n0 * pc.p * csz;
+  const size_t ptab = (size_t)pencil_table_entries(pc) * csz;
   const size_t budget = std::min<size_t>(pl.tuning.lds_budget, 160 * 1024);
   int W = pc.w;
-  while (W > 1 && per_w * W > budget) W /= 2;
-  if (per_w * W > budget)
+  while (W > 1 && per_w * W + ptab > budget) W /= 2;
+  if (per_w * W + ptab > budget)
     return fail(2, "the pencil tile of this grid does not fit the LDS budget with " +
                        std::to_string(cmax) + " components");
   pc.w = W;
@@ -478,7 +490,7 @@ Status mode_geometry(const Plan& pl, double mu, double nu, int mode, ModeGeometr
   nthr = std::max(nthr, round_up(pc.ax0.tpt, 64));
   if (!fused_split(pc.p)) nthr = std::max(nthr, round_up(pc.pfft.tpt, 64));
   g.pencil_threads = nthr;
-  g.pencil_lds = per_w * W;
+  g.pencil_lds = per_w * W + ptab;
   g.ws_bytes = pl.ws_bytes / pl.dim * cmax;
   return Status{};
 }
