@@ -99,6 +99,22 @@ def _traffic(name):
         return None
 
 
+def _valu(name, kernel_ms):
+    """VALU issue time of kernel `name` per launch, from the committed SQ
+    counter summary (SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz)), as a
+    fraction of its measured duration."""
+    p = os.path.join(ROOT, "profiles", "sq_256cube_f32.json")
+    try:
+        with open(p) as f:
+            d = json.load(f)
+        k = d["kernels"][name]
+    except (OSError, KeyError, ValueError):
+        return None
+    return {"insts_per_launch": k["SQ_INSTS_VALU"], "issue_us": k["valu_issue_us"],
+            "kernel_us": 1e3 * kernel_ms, "frac": k["valu_issue_us"] / (1e3 * kernel_ms),
+            "model": d["model"]}
+
+
 def _host_cpu():
     """CPU of this host: model name, logical CPUs, and the CPUs this process
     may run on."""
@@ -379,7 +395,16 @@ def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
     st = srv.stats()
     srv.close()
     lat = np.array([r["latency_s"] for r in res]) * 1e3
+    qd = np.array([r["queued_s"] for r in res]) * 1e3
     ok = sum(r["state"] == "done" for r in res)
+    sides = np.array([q[0] for q in mix])
+    by_side = {}
+    for n in SIDES_SERVE:
+        sel = sides == n
+        if sel.sum() >= 10:
+            by_side[str(n)] = {"requests": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                               "p99_ms": float(np.percentile(lat[sel], 99)),
+                               "mean_queued_ms": float(qd[sel].mean())}
     waves = st["waves"] - st0["waves"]
     nodes = sum(n ** 3 for n, *_ in mix)
     return {"requests": n_requests, "completed": ok, "arrival_rate_hz": rate, "load": load,
@@ -387,6 +412,7 @@ def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
             "throughput_Mnodes_per_s": nodes / (t_end - t0) / 1e6,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+            "mean_queued_ms": float(qd.mean()), "by_side": by_side,
             "waves": waves, "mean_wave_size": n_requests / max(waves, 1),
             "peak_waves_in_flight": st["peak_waves_in_flight"], "hbm_budget": st["hbm_budget"],
             "hbm_peak": st["hbm_peak"], "plans_resident": st["plans_resident"],
@@ -514,7 +540,9 @@ def main():
         "config": {"workload": "3D H8 periodic stiffness K.u, %d^3 unit cube, mu=1, nu=0.25" % n,
                    "grid": list(shape), "components": 3, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(dom)},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(dom),
+                     "limiter": "valu issue + latency (one workgroup per CU)",
+                     "valu": _valu(dom, per_kernel[dom]) if n == 256 else None},
         "per_kernel_ms": per_kernel,
         "apply_roofline": {"algorithmic_bytes": b_alg, "achieved_GBs": b_alg / (ms_step * 1e-3) / 1e9,
                            "frac": b_alg / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
