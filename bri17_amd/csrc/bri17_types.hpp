@@ -30,9 +30,11 @@ struct PassDesc {
 };
 
 // A length-n transform done by tpt threads per sequence.  Twiddles are read
-// as tw[m * tws] from a table of exp(-2 pi i m / (n tws)).
+// as tw[m * tws] from a table of exp(-2 pi i m / (n tws)).  wave: tpt is a
+// power of two up to 64 and a sequence's threads are lanes of one wave, which
+// orders the passes' LDS exchanges without workgroup barriers.
 struct FftDesc {
-  int n, npass, tpt, tws;
+  int n, npass, tpt, tws, wave;
   PassDesc pass[kMaxPass];
 };
 

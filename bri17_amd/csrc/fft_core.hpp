@@ -82,6 +82,61 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv d) {
   return (__umulhi(n, d.m) + n) >> d.s;
 }
 
+// Orders one wave's LDS accesses around it for the compiler (a wavefront-
+// scope fence and wave barrier; the hardware executes a wave's LDS
+// instructions in order).  An exchange whose writers and readers all lie in
+// one wave needs no workgroup barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Barrier between the read and write halves of a Stockham pass: a wave
+// barrier when every thread of each sequence lies in one wave (`wave`,
+// workgroup-uniform), else a workgroup barrier.
+__device__ __forceinline__ void pass_sync(bool wave) {
+  if (wave)
+    wave_lds_sync();
+  else
+    __syncthreads();
+}
+
+// Threads of a stage of equal-length sequences of descriptor D, D.tpt per
+// sequence.  D.wave (a power-of-two tpt up to 64): a sequence's threads are
+// lanes of one wave, with the sequence index fastest across the lanes of a
+// wave, and the stage's passes synchronise per wave.  Otherwise, if
+// `seq_fastest`, the sequence index is fastest across all threads (LDS
+// accesses of neighbouring threads hit neighbouring sequences, for
+// sequences strided in LDS), else a sequence's threads are consecutive.
+// S: sequences per round of the workgroup.
+struct SeqThreads {
+  int sl, lane, S;
+  bool wave;
+};
+__device__ __forceinline__ SeqThreads seq_threads(int tid, int nthr, const FftDesc& D,
+                                                  bool seq_fastest) {
+  SeqThreads m;
+  const int tpt = D.tpt;
+  m.wave = D.wave != 0;
+  if (m.wave) {
+    const int spw = 64 / tpt;  // sequences per wave
+    const int li = tid & 63;
+    m.sl = (tid >> 6) * spw + li % spw;
+    m.lane = li / spw;
+    m.S = (nthr >> 6) * spw;
+  } else if (seq_fastest) {
+    m.S = nthr / tpt;
+    m.sl = tid % m.S;
+    m.lane = tid / m.S;
+  } else {
+    m.sl = tid / tpt;
+    m.lane = tid - m.sl * tpt;
+    m.S = nthr / tpt;
+  }
+  return m;
+}
+
 // ---------------------------------------------------------------- butterflies
 
 template <bool INV, typename T>
@@ -281,7 +336,7 @@ template <typename T, int R, bool INV, int MAXB, typename TWP>
 __device__ __forceinline__ void fft_pass(cpx<T>* buf, TWP tw,
                                          const PassDesc& P, int N, int tws,
                                          int base, int es, int lane, int tpt,
-                                         bool active) {
+                                         bool active, bool wave) {
   const int NR = P.nr;
   const int Ns = P.ns;
   const int twm = P.twm * tws;
@@ -309,7 +364,7 @@ __device__ __forceinline__ void fft_pass(cpx<T>* buf, TWP tw,
       }
     }
   }
-  __syncthreads();
+  pass_sync(wave);
   if (active) {
 #pragma unroll
     for (int i = 0; i < MAXB; i++) {
@@ -320,43 +375,44 @@ __device__ __forceinline__ void fft_pass(cpx<T>* buf, TWP tw,
       }
     }
   }
-  __syncthreads();
+  pass_sync(wave);
 }
 
 // Full FFT of length D.n on the LDS sequence at `base` (stride es).  Every
-// thread of the workgroup must call this (barriers inside); `active` marks
-// threads that own a sequence.
+// thread of the workgroup (`wave`: of the wave, whose threads then hold
+// whole sequences; see seq_threads) must call this (barriers inside);
+// `active` marks threads that own a sequence.
 // `tw` is a pointer to const cpx<T> in any address space.
 template <typename T, bool INV, int MAXE, typename TWP>
 __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
                                         const FftDesc& D, int base, int es,
-                                        int lane, bool active) {
+                                        int lane, bool active, bool wave = false) {
   for (int p = 0; p < D.npass; p++) {
     const PassDesc& P = D.pass[p];
     switch (P.r) {
       case 2:
-        fft_pass<T, 2, INV, MAXE / 2>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 2, INV, MAXE / 2>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 3:
-        fft_pass<T, 3, INV, MAXE / 3>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 3, INV, MAXE / 3>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 4:
-        fft_pass<T, 4, INV, MAXE / 4>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 4, INV, MAXE / 4>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 5:
-        fft_pass<T, 5, INV, MAXE / 5>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 5, INV, MAXE / 5>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 7:
-        fft_pass<T, 7, INV, MAXE / 7>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 7, INV, MAXE / 7>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 8:
-        fft_pass<T, 8, INV, MAXE / 8>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 8, INV, MAXE / 8>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 9:
-        fft_pass<T, 9, INV, MAXE / 9>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 9, INV, MAXE / 9>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 16:
-        fft_pass<T, 16, INV, MAXE / 16>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active);
+        fft_pass<T, 16, INV, MAXE / 16>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       default:
         break;

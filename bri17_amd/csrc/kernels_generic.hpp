@@ -121,15 +121,15 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
   }
   __syncthreads();
 
-  // 2. row FFTs, consecutive threads on consecutive elements of a row
+  // 2. row FFTs (a row's threads within one wave where possible)
   {
-    const int tpt = pp.row.tpt, S = nthr / tpt;
-    const int sl = tid / tpt, lane = tid - sl * tpt;
-    for (int r0 = 0; r0 < nrows; r0 += S) {
-      const int s = r0 + sl;
-      const bool act = sl < S && s < nrows;
-      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, lane, act);
+    const SeqThreads m = seq_threads(tid, nthr, pp.row, false);
+    for (int r0 = 0; r0 < nrows; r0 += m.S) {
+      const int s = r0 + m.sl;
+      const bool act = m.sl < m.S && s < nrows;
+      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, m.lane, act, m.wave);
     }
+    if (m.wave) __syncthreads();
   }
 
   // 3. split the packed transform into the nl/2+1 spectrum entries
@@ -156,14 +156,13 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
   // 4. 3D: Q-point stage of the middle axis on every column, then the split
   //    twiddle w_N1^(p ka) is applied on the way out
   if (pp.has_col) {
-    const int tpt = pp.col.tpt, S = nthr / tpt;
-    const int sl = tid % S, lane = tid / S;
-    const bool inr = tid < S * tpt;
-    for (int c0 = 0; c0 < pp.nlh; c0 += S) {
-      const int s = c0 + sl;
-      const bool act = inr && s < pp.nlh;
-      lds_fft<T, false, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, lane, act);
+    const SeqThreads m = seq_threads(tid, nthr, pp.col, true);
+    for (int c0 = 0; c0 < pp.nlh; c0 += m.S) {
+      const int s = c0 + m.sl;
+      const bool act = m.sl < m.S && s < pp.nlh;
+      lds_fft<T, false, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, m.lane, act, m.wave);
     }
+    if (m.wave) __syncthreads();
   }
 
   // 5. store
@@ -222,14 +221,13 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
 
   // 2. 3D: inverse Q-point stage of the middle axis
   if (pp.has_col) {
-    const int tpt = pp.col.tpt, S = nthr / tpt;
-    const int sl = tid % S, lane = tid / S;
-    const bool inr = tid < S * tpt;
-    for (int c0 = 0; c0 < pp.nlh; c0 += S) {
-      const int s = c0 + sl;
-      const bool act = inr && s < pp.nlh;
-      lds_fft<T, true, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, lane, act);
+    const SeqThreads m = seq_threads(tid, nthr, pp.col, true);
+    for (int c0 = 0; c0 < pp.nlh; c0 += m.S) {
+      const int s = c0 + m.sl;
+      const bool act = m.sl < m.S && s < pp.nlh;
+      lds_fft<T, true, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, m.lane, act, m.wave);
     }
+    if (m.wave) __syncthreads();
   }
 
   // 3. rebuild the transform the row FFT needs
@@ -265,13 +263,13 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
 
   // 4. inverse row FFTs
   {
-    const int tpt = pp.row.tpt, S = nthr / tpt;
-    const int sl = tid / tpt, lane = tid - sl * tpt;
-    for (int r0 = 0; r0 < nrows; r0 += S) {
-      const int s = r0 + sl;
-      const bool act = sl < S && s < nrows;
-      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, lane, act);
+    const SeqThreads m = seq_threads(tid, nthr, pp.row, false);
+    for (int r0 = 0; r0 < nrows; r0 += m.S) {
+      const int s = r0 + m.sl;
+      const bool act = m.sl < m.S && s < nrows;
+      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, m.lane, act, m.wave);
     }
+    if (m.wave) __syncthreads();
   }
 
   // 5. store real rows
@@ -337,16 +335,15 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
 template <typename T, bool INV>
 __device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const cpx<T>* tw1, const PencilParams& pp,
                                               int nouter, int tid, int nthr) {
-  const int W = pp.w, P = pp.p, tpt = pp.pfft.tpt, S = nthr / tpt;
-  const int sl = tid % S, lane = tid / S;
-  const bool inr = tid < S * tpt;
+  const int W = pp.w, P = pp.p;
+  const SeqThreads m = seq_threads(tid, nthr, pp.pfft, true);
   const int nseq = nouter * W;
-  for (int s0 = 0; s0 < nseq; s0 += S) {
-    const int s = s0 + sl;
-    const bool act = inr && s < nseq;
+  for (int s0 = 0; s0 < nseq; s0 += m.S) {
+    const int s = s0 + m.sl;
+    const bool act = m.sl < m.S && s < nseq;
     const int outer = act ? (int)fdiv((uint32_t)s, pp.wdiv) : 0;
     const int base = act ? outer * P * W + (s - outer * W) : 0;
-    lds_fft<T, INV, kMaxE>(lds, tw1, pp.pfft, base, W, lane, act);
+    lds_fft<T, INV, kMaxE>(lds, tw1, pp.pfft, base, W, m.lane, act, m.wave);
   }
 }
 
@@ -392,16 +389,15 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     __syncthreads();  // the axis-0 passes read columns other threads wrote
   }
 
-  const int tpt = pp.ax0.tpt, S = nthr / tpt;
-  const int sl = tid % S, lane = tid / S;
-  const bool inr = tid < S * tpt;
-  for (int s0 = 0; s0 < CIN * PW; s0 += S) {
-    const int s = s0 + sl;
-    const bool act = inr && s < CIN * PW;
+  const SeqThreads m = seq_threads(tid, nthr, pp.ax0, true);
+  for (int s0 = 0; s0 < CIN * PW; s0 += m.S) {
+    const int s = s0 + m.sl;
+    const bool act = m.sl < m.S && s < CIN * PW;
     const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
     const int base = act ? g * cst + (s - g * PW) : 0;
-    lds_fft<T, false, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
+    lds_fft<T, false, kMaxE>(lds, tw0, pp.ax0, base, PW, m.lane, act, m.wave);
   }
+  if (m.wave) __syncthreads();
 
   {
     const T mu = T(pp.mu), sc = T(pp.s), kinv = T(pp.kinv);
@@ -437,13 +433,14 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
   }
   __syncthreads();
 
-  for (int s0 = 0; s0 < COUT * PW; s0 += S) {
-    const int s = s0 + sl;
-    const bool act = inr && s < COUT * PW;
+  for (int s0 = 0; s0 < COUT * PW; s0 += m.S) {
+    const int s = s0 + m.sl;
+    const bool act = m.sl < m.S && s < COUT * PW;
     const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
     const int base = act ? g * cst + (s - g * PW) : 0;
-    lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, lane, act);
+    lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, m.lane, act, m.wave);
   }
+  if (m.wave) __syncthreads();
   if (P > 1) {
     if (fused_split(P))
       pdft<T, true>(lds, P, W, pp.wdiv, COUT * N0, tid, nthr);

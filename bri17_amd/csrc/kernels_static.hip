@@ -45,17 +45,9 @@ __device__ __forceinline__ cpx<T> s_pre_c2r_pmh(cpx<T> A, cpx<T> B, T p, T m, T 
           fmat(A.y, p, fmat(-B.y, m, fmat(A.x, h, -(B.x * h))))};
 }
 
-// Orders one wave's LDS accesses around it for the compiler (a wavefront-
-// scope fence and wave barrier; the hardware executes a wave's LDS
-// instructions in order).  An exchange whose writers and readers all lie in
-// one wave needs no workgroup barrier: the plane kernels' row exchanges
-// (the M/R1 threads of a row are consecutive lanes of one wave) and column
-// exchanges (see col_cs).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
+// wave_lds_sync (fft_core.hpp) orders the plane kernels' row exchanges (the
+// M/R1 threads of a row are consecutive lanes of one wave) and column
+// exchanges (see col_cs) without a workgroup barrier.
 
 // Column-stage thread mapping of the plane kernels: the TPC threads of a
 // column are lanes of one wave (64 / TPC consecutive columns per wave, column

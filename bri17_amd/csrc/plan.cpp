@@ -43,13 +43,14 @@ std::vector<int> default_radices(int n) {
 }
 
 bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
-                   std::string* why) {
+                   std::string* why, bool want_wave) {
   std::memset(&d, 0, sizeof(d));
   d.n = n;
   d.tws = tws;
   if (n == 1) {
     d.npass = 0;
     d.tpt = 1;
+    d.wave = 0;
     return true;
   }
   int prod = 1, rmax = 1;
@@ -76,6 +77,19 @@ bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
       if (per * r > kMaxE) ok = false;
     }
     if (ok) break;
+  }
+  // Wave-local passes (seq_threads): the threads of a sequence fit one wave
+  // when tpt is a power of two up to 64; tpt is rounded up to one where that
+  // idles at most a quarter of the lanes.  The caller decides whether the
+  // stage may use it (LDS strides of its sequences).
+  d.wave = 0;
+  if (want_wave) {
+    int t2 = 1;
+    while (t2 < tpt) t2 *= 2;
+    if (t2 <= 64 && 4 * tpt >= 3 * t2) {
+      tpt = t2;
+      d.wave = 1;
+    }
   }
   d.tpt = tpt;
   d.npass = (int)radices.size();
@@ -209,7 +223,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   pp.m = m;
   pp.rsl = rsl;
   pp.rs = nlh;
-  if (!make_fft_desc(m, nl / m, pick(tun, 0, default_radices(m)), pp.row, &why))
+  if (!make_fft_desc(m, nl / m, pick(tun, 0, default_radices(m)), pp.row, &why, true))
     return fail(2, "row FFT: " + why);
   pp.mdiv = make_fastdiv((uint32_t)m);
   pp.hdiv = make_fastdiv((uint32_t)(m / 2 + 1));
@@ -259,7 +273,9 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     pp.p = P;
     pp.n1 = n1;
     pp.has_col = 1;
-    if (!make_fft_desc(Q, P, pick(tun, 1, default_radices(Q)), pp.col, &why))
+    // wave-local column passes step through a column at the row stride
+    // rsl: conflict-free LDS banks need it odd
+    if (!make_fft_desc(Q, P, pick(tun, 1, default_radices(Q)), pp.col, &why, rsl % 2 == 1))
       return fail(2, "middle-axis FFT: " + why);
     int nthr = std::max(Q * pp.row.tpt, nlh * pp.col.tpt);
     nthr = std::min(1024, round_up(nthr, 64));

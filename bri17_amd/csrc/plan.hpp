@@ -36,8 +36,9 @@ struct Tuning {
 // Radix order (largest first) for a Stockham transform of length n, or empty
 // if n has a prime factor > 7.
 std::vector<int> default_radices(int n);
+// want_wave: the stage may run wave-local passes (FftDesc::wave).
 bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
-                   std::string* why);
+                   std::string* why, bool want_wave = false);
 
 struct Plan {
   int dim = 0, dtype = 0;
