@@ -329,11 +329,12 @@ def batched(dev, steps, warmup, seed=4):
             "ms_sequential_calls": 1e3 * t_seq, "speedup_vs_sequential": t_seq / t, "launch": b.info()}
 
 
-def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
+def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5, server_kw=None, rate=None):
     """Poisson arrivals of fp32 3D requests, side log-uniform in [32,256]
     rounded (in log space) to 2^a 3^b, 70% K.u / 30% Green solve, mu ~
     U[0.5,2], nu ~ U[0,0.45].  The arrival rate is `load` times the capacity
-    measured by a closed-loop burst.  Inputs stay resident in HBM (one per
+    measured by a closed-loop burst (or `rate`, requests/s, when given);
+    `server_kw` are Server options.  Inputs stay resident in HBM (one per
     side, read-only); outputs come from a ring per side whose slots are
     reused only after their previous request finished."""
     import numpy as np
@@ -350,7 +351,7 @@ def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
     ring = 16
     outs = {n: [torch.empty_like(inputs[n]) for _ in range(ring)] for n in SIDES_SERVE}
     torch.cuda.synchronize()
-    srv = Server()
+    srv = Server(**(server_kw or {}))
     holder = {n: [None] * ring for n in SIDES_SERVE}
     count = {n: 0 for n in SIDES_SERVE}
 
@@ -381,7 +382,7 @@ def serving(dev, n_requests=2000, n_capacity=400, load=0.7, seed=5):
             r.wait()
         cap = max(cap, n_capacity / (time.perf_counter() - t0))
     # open loop: Poisson arrivals at load * capacity
-    rate = load * cap
+    rate = rate or load * cap
     arr = poisson_arrivals(rate, n_requests, seed=seed + 1)
     mix = draw(n_requests)
     st0 = srv.stats()

@@ -118,7 +118,8 @@ class ServerConfig(ctypes.Structure):
     """bri17_server_config of include/bri17_amd.h."""
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("max_batch", ctypes.c_int),
                 ("max_waves", ctypes.c_int), ("solo_nodes", ctypes.c_int64),
-                ("spin_limit", ctypes.c_int)]
+                ("spin_limit", ctypes.c_int), ("heavy_nodes", ctypes.c_int64),
+                ("max_heavy_waves", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):

@@ -10,7 +10,9 @@
 // report their dependency-wait status word), returning their buffers to the
 // idle pool; (2) while a stream slot is free, take requests from the head of
 // the queue -- one request with fixed-size kernels or a large grid alone, or
-// a run of consecutive batchable requests of one dtype -- reserve their HBM
+// a run of consecutive batchable requests of one dtype; a heavy request
+// (large grid) whose turn comes while `max_heavy_waves` heavy waves are in
+// flight stays queued and the requests behind it go first -- reserve their HBM
 // footprint (idle buffers first, then new allocations, evicting idle buffers
 // when the budget requires) and launch them; (3) when nothing is in flight,
 // sleep until a submission arrives, else poll.
@@ -58,6 +60,7 @@ struct Req {
   size_t ws_cls = 0, stage_cls = 0;  // stage_cls: class of each staged field (host memory)
   size_t footprint = 0;
   bool solo = false;
+  bool heavy = false;
   int state = BRI17_REQ_QUEUED;
   int error = 0;
   Clock::time_point t_submit, t_launch, t_done;
@@ -76,6 +79,7 @@ struct Slot {
   int* h_status = nullptr;  // pinned copy of the wave's dependency-wait status
   bool busy = false;
   bool batched = false;
+  bool heavy = false;
   int launch_error = 0;
   std::vector<std::shared_ptr<Req>> reqs;
 };
@@ -309,26 +313,34 @@ bool busy_any(const bri17_server* s) {
   return false;
 }
 
-// Take the next wave from the queue head into `sl` and launch it.  false:
-// nothing launched (queue empty or the head does not fit now).
+// Take the next wave from the queue into `sl` and launch it: the first
+// request that may start now (a heavy one only while fewer than
+// max_heavy_waves heavy waves are in flight), with the batchable requests
+// right behind it.  false: nothing launched (nothing may start, or the
+// request does not fit in the HBM budget now).
 bool start_wave(bri17_server* s, Slot& sl) {
+  int heavy_in_flight = 0;
+  for (auto& o : s->slots) heavy_in_flight += o.busy && o.heavy ? 1 : 0;
+  const bool heavy_ok = s->cfg.max_heavy_waves < 0 || heavy_in_flight < s->cfg.max_heavy_waves;
   std::vector<std::shared_ptr<Req>> take;
+  size_t at = 0;  // queue position of the first request taken
   {
     std::lock_guard<std::mutex> g(s->mtx);
-    if (s->queue.empty()) return false;
-    const auto head = s->queue.front();
-    take.push_back(head);
-    s->queue.pop_front();
-    if (!head->solo) {
-      while (!s->queue.empty() && (int)take.size() < s->cfg.max_batch) {
-        const auto& nx = s->queue.front();
-        if (nx->solo || nx->r.dtype != head->r.dtype) break;
-        take.push_back(nx);
-        s->queue.pop_front();
+    while (at < s->queue.size() && s->queue[at]->heavy && !heavy_ok) at++;
+    if (at == s->queue.size()) return false;
+    const auto first = s->queue[at];
+    size_t end = at + 1;
+    if (!first->solo) {
+      while (end < s->queue.size() && (int)(end - at) < s->cfg.max_batch) {
+        const auto& nx = s->queue[end];
+        if (nx->solo || nx->r.dtype != first->r.dtype) break;
+        end++;
       }
     }
+    take.assign(s->queue.begin() + (std::ptrdiff_t)at, s->queue.begin() + (std::ptrdiff_t)end);
+    s->queue.erase(s->queue.begin() + (std::ptrdiff_t)at, s->queue.begin() + (std::ptrdiff_t)end);
   }
-  // reserve footprints in order; what does not fit goes back to the head
+  // reserve footprints in order; what does not fit goes back where it was
   const bool others = busy_any(s);
   size_t n_ok = 0;
   int fail_code = 0;
@@ -342,7 +354,7 @@ bool start_wave(bri17_server* s, Slot& sl) {
     std::lock_guard<std::mutex> g(s->mtx);
     size_t back_from = n_ok;
     if (fail_code && n_ok == 0) {
-      // the head cannot be allocated even with an idle device: fail it
+      // the request cannot be allocated even with an idle device: fail it
       auto& q = take[0];
       q->state = BRI17_REQ_FAILED;
       q->t_done = Clock::now();
@@ -351,12 +363,14 @@ bool start_wave(bri17_server* s, Slot& sl) {
       back_from = 1;
       s->cv_done.notify_all();
     }
-    for (size_t i = take.size(); i-- > back_from;) s->queue.push_front(take[i]);
+    s->queue.insert(s->queue.begin() + (std::ptrdiff_t)at, take.begin() + (std::ptrdiff_t)back_from,
+                    take.end());
     take.resize(n_ok);
   }
   if (take.empty()) return fail_code != 0;
   sl.reqs = std::move(take);
   sl.batched = !sl.reqs[0]->solo;
+  sl.heavy = sl.reqs[0]->heavy;
   sl.busy = true;
   int64_t in_flight = 0;
   for (auto& o : s->slots) in_flight += o.busy ? 1 : 0;
@@ -450,6 +464,8 @@ int bri17_server_create(const bri17_server_config* cfg, bri17_server** out) {
   if (c.max_waves > 16) return b17_err(BRI17_ERR_INVALID, "max_waves is at most 16");
   if (c.solo_nodes == 0) c.solo_nodes = (int64_t)1 << 21;
   if (c.spin_limit <= 0) c.spin_limit = (int)kDefaultSpin;
+  if (c.heavy_nodes <= 0) c.heavy_nodes = (int64_t)1 << 22;
+  if (c.max_heavy_waves == 0) c.max_heavy_waves = 1;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return b17_err(BRI17_ERR_HIP, "no HIP device");
   if (c.hbm_budget_bytes == 0) {
@@ -573,6 +589,7 @@ int bri17_server_submit_many(bri17_server* s, const bri17_request* reqs, int n, 
       break;
     }
     q->solo = pl.use_static || (s->cfg.solo_nodes > 0 && pl.nnodes >= s->cfg.solo_nodes);
+    q->heavy = pl.nnodes >= s->cfg.heavy_nodes;
     qs[i] = std::move(q);
   }
   if (prev >= 0 && prev != s->device) (void)hipSetDevice(prev);

@@ -4,8 +4,11 @@ Python face of ``bri17_server_*`` (include/bri17_amd.h).  Requests of mixed
 grids, dtypes, materials and modes are queued from any thread; the library's
 worker thread admits them in arrival order against an HBM budget and runs
 them in waves on several streams while earlier waves are still on the
-device.  No reference counterpart: the reference applies one operator per
-call on the CPU (tests/test_bri17.cpp:56-107).
+device.  Heavy requests (``heavy_nodes`` nodes or more, default 2**22) run
+at most ``max_heavy_waves`` at a time (default 1, -1 for no limit); the
+requests behind a heavy one that has to wait for that start first.  No
+reference counterpart: the reference applies one operator per call on the
+CPU (tests/test_bri17.cpp:56-107).
 
 Device tensors are read on the server's own streams, so their producer must
 have finished when ``submit`` is called (``sync=True`` synchronises torch's
@@ -63,9 +66,10 @@ class ServedRequest:
 
 
 class Server:
-    def __init__(self, hbm_budget_bytes=0, max_batch=0, max_waves=0, solo_nodes=0, spin_limit=0):
+    def __init__(self, hbm_budget_bytes=0, max_batch=0, max_waves=0, solo_nodes=0, spin_limit=0,
+                 heavy_nodes=0, max_heavy_waves=0):
         cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(max_batch), int(max_waves), int(solo_nodes),
-                                int(spin_limit))
+                                int(spin_limit), int(heavy_nodes), int(max_heavy_waves))
         h = ctypes.c_void_p()
         _lib.check(_lib.lib().bri17_server_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -273,7 +273,10 @@ int bri17_operator_profile_read(bri17_operator* op, int* calls, double* ms3);
  * mixed grids, dtypes, materials and modes are submitted from any thread; a
  * worker thread admits them in arrival order against an HBM budget and
  * launches them in waves, each wave on its own stream, while earlier waves
- * still run.  A wave is either one persistent batched launch of requests
+ * still run.  Heavy requests (grids of at least `heavy_nodes` nodes) run at
+ * most `max_heavy_waves` at a time; a heavy request that has to wait for that
+ * lets the requests behind it start, so small grids are not held behind
+ * several large ones.  A wave is either one persistent batched launch of requests
  * with runtime-size plans (bri17_batch_*), or one request that runs alone on
  * its plan's own kernels (grids with fixed-size kernels, and grids of at
  * least `solo_nodes` nodes).  Which of the two a request takes depends on the
@@ -294,6 +297,10 @@ typedef struct bri17_server_config {
   int64_t solo_nodes;      /* grids of at least this many nodes run alone
                               (0: 2^21, -1: never) */
   int spin_limit;          /* dependency-wait bound of batched waves (0: default) */
+  int64_t heavy_nodes;     /* grids of at least this many nodes are heavy (0: 2^22) */
+  int max_heavy_waves;     /* heavy waves in flight at once (0: 1, -1: no limit);
+                              while the limit is reached, a queued heavy request
+                              lets the requests behind it start first */
 } bri17_server_config;
 
 #define BRI17_MEM_DEVICE 0 /* in / out are device pointers owned by the caller */

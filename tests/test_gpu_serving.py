@@ -124,6 +124,31 @@ def test_alone_vs_mix_bitwise(device, oracle, mix):
         assert err <= tol
 
 
+def test_heavy_requests_let_small_ones_pass(device):
+    """Grids of at least heavy_nodes (2^22) nodes run one at a time by
+    default; while one does, the small requests queued behind the other
+    heavy ones start first.  With no limit the waves start in arrival
+    order.  Either way every result is the same, bitwise."""
+    import torch
+    from bri17_amd import Server
+    spec = [dict(shape=(n,) * 3, dtype="float32", mode=m, L=(1.0, 1.0, 1.0), mu=1.1, nu=0.3)
+            for n, m in ((162, "apply"), (192, "solve"), (162, "apply"), (48, "apply"), (36, "solve"))]
+    xs = _inputs(device, spec, seed=21)
+    with Server(max_batch=8, max_waves=4) as srv:
+        out, res = _serve(srv, spec, xs)
+    with Server(max_batch=8, max_waves=4, max_heavy_waves=-1) as srv:
+        out_fifo, res_fifo = _serve(srv, spec, xs)
+    waves = [r["wave"] for r in res]
+    waves_fifo = [r["wave"] for r in res_fifo]
+    print("waves (heavy limit 1):", waves, " (no limit):", waves_fifo, flush=True)
+    assert all(r["state"] == "done" for r in res + res_fifo)
+    assert waves[0] < waves[1] < waves[2]
+    assert max(waves[3:]) < waves[1], "the small requests waited behind a heavy one"
+    assert waves_fifo[0] < waves_fifo[1] < waves_fifo[2] < min(waves_fifo[3:])
+    for a, b in zip(out, out_fifo):
+        assert torch.equal(a, b)
+
+
 def test_host_memory_requests_match_device_requests(device):
     import torch
     from bri17_amd import Server
