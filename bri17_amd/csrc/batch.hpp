@@ -19,10 +19,12 @@ struct BatchInst {
 };
 
 // Work item encoding: x = inst | type << 16 (0 plane fwd, 1 pencil,
-// 2 plane inv), y = block / tile index within the instance.
-// ctl: [head, error, done(2 per instance)], nctl = 2 + 2 * ninst.
-hipError_t launch_batch(int dtype, const BatchInst* insts, const int2* items, int nitems, int* ctl,
-                        int nctl, int threads, size_t lds, int spin_limit, hipStream_t st,
-                        int* grid_out);
+// 2 plane inv), y = block / tile index within the instance.  n_fwd, n_pen:
+// forward items and pencil tiles of all instances.
+// ctl: [head, error, forward items done, pencil tiles done, done(2 per
+// instance)], nctl = 4 + 2 * ninst.
+hipError_t launch_batch(int dtype, const BatchInst* insts, const int2* items, int nitems,
+                        int n_fwd, int n_pen, int* ctl, int nctl, int threads, size_t lds,
+                        int spin_limit, hipStream_t st, int* grid_out);
 
 }  // namespace b17

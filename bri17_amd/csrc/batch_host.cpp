@@ -24,6 +24,7 @@ struct bri17_batch {
   int nctl = 0;
   int spin_limit = 1 << 24;
   int last_nitems = 0, last_grid = 0, last_threads = 0, last_dtype = 0;
+  int last_nfwd = 0, last_npen = 0;
   size_t last_lds = 0, last_off_items = 0;
   int graphs = 0;  // live graphs captured from this context (they replay its tables)
   std::mutex mtx;
@@ -158,7 +159,12 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
       for (int k = 0; k < cnt; k++) work.push_back(make_int2(i | (type << 16), k));
     }
   const int nitems = (int)work.size();
-  const int nctl = 2 + 2 * n;
+  const int nctl = 4 + 2 * n;
+  int n_fwd = 0, n_pen = 0;
+  for (int i = 0; i < n; i++) {
+    n_fwd += insts[i].na;
+    n_pen += insts[i].nb;
+  }
   const size_t off_items = align_up(sizeof(b17::BatchInst) * n, 256);
   const size_t off_ctl = align_up(off_items + sizeof(int2) * work.size(), 256);
   const size_t bytes = off_ctl + sizeof(int) * nctl;
@@ -183,13 +189,15 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
   b->nctl = nctl;
   int grid = 0;
   hipError_t e = b17::launch_batch(dtype, reinterpret_cast<const b17::BatchInst*>(d),
-                                   reinterpret_cast<const int2*>(d + off_items), nitems, b->d_ctl,
-                                   nctl, threads, lds, b->spin_limit, st, &grid);
+                                   reinterpret_cast<const int2*>(d + off_items), nitems, n_fwd,
+                                   n_pen, b->d_ctl, nctl, threads, lds, b->spin_limit, st, &grid);
   if (e != hipSuccess) return b17_err(BRI17_ERR_HIP, hipGetErrorString(e));
   if (hipEventRecord(b->done, st) != hipSuccess)
     return b17_err(BRI17_ERR_HIP, "hipEventRecord(batch) failed");
   b->pending = true;
   b->last_nitems = nitems;
+  b->last_nfwd = n_fwd;
+  b->last_npen = n_pen;
   b->last_grid = grid;
   b->last_threads = threads;
   b->last_lds = lds;
@@ -226,8 +234,8 @@ int b17_batch_relaunch(bri17_batch* b, hipStream_t st) {
   int grid = 0;
   hipError_t e = b17::launch_batch(b->last_dtype, reinterpret_cast<const b17::BatchInst*>(d),
                                    reinterpret_cast<const int2*>(d + b->last_off_items),
-                                   b->last_nitems, b->d_ctl, b->nctl, b->last_threads, b->last_lds,
-                                   b->spin_limit, st, &grid);
+                                   b->last_nitems, b->last_nfwd, b->last_npen, b->d_ctl, b->nctl,
+                                   b->last_threads, b->last_lds, b->spin_limit, st, &grid);
   if (e != hipSuccess) return b17_err(BRI17_ERR_HIP, hipGetErrorString(e));
   return BRI17_OK;
 }
