@@ -100,34 +100,42 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
   cpx<T>* const twc = twr + pp.nl;
   copy_tables<T>(pp, twr, twc, tid, nthr);
 
-  // 1. rows -> LDS (even rows are read as m complex pairs)
+  // 1. rows -> LDS.  Even lengths: a row is read as m complex pairs.  Odd
+  //    lengths: rows 2j and 2j+1 are read as the real and imaginary parts of
+  //    one complex row in slot 2j (one transform for two rows).
   {
-    const int m = pp.m, tot = nrows * m, rsl = pp.rsl;
-    auto put = [&](int e, cpx<T> v) {
-      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      lds[qq * rsl + (e - qq * m)] = v;
-    };
+    const int m = pp.m, rsl = pp.rsl;
     if (pp.even) {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
+      staged_copy<kLoads<T>, cpx<T>>(nrows * m, tid, nthr, [&](int e) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
         return ((gptr<const cpx<T>>)(u + (row0 + (int64_t)qq * pst) * pp.nl))[e - qq * m];
-      }, put);
-    } else {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, [&](int e) {
+      }, [&](int e, cpx<T> v) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        return cpx<T>{u[(row0 + (int64_t)qq * pst) * pp.nl + (e - qq * m)], T(0)};
-      }, put);
+        lds[qq * rsl + (e - qq * m)] = v;
+      });
+    } else {
+      staged_copy<kLoads<T>, cpx<T>>(((nrows + 1) / 2) * m, tid, nthr, [&](int e) {
+        const int j = (int)fdiv((uint32_t)e, pp.mdiv);
+        const int n = e - j * m;
+        const gptr<const T> ra = u + (row0 + (int64_t)(2 * j) * pst) * pp.nl;
+        return cpx<T>{ra[n], 2 * j + 1 < nrows ? ra[(int64_t)pst * pp.nl + n] : T(0)};
+      }, [&](int e, cpx<T> v) {
+        const int j = (int)fdiv((uint32_t)e, pp.mdiv);
+        lds[2 * j * rsl + (e - j * m)] = v;
+      });
     }
   }
   __syncthreads();
 
   // 2. row FFTs (a row's threads within one wave where possible)
   {
+    const int nseq = pp.even ? nrows : (nrows + 1) / 2;
+    const int sst = pp.even ? pp.rsl : 2 * pp.rsl;
     const SeqThreads m = seq_threads(tid, nthr, pp.row, false);
-    for (int r0 = 0; r0 < nrows; r0 += m.S) {
+    for (int r0 = 0; r0 < nseq; r0 += m.S) {
       const int s = r0 + m.sl;
-      const bool act = m.sl < m.S && s < nrows;
-      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, m.lane, act, m.wave);
+      const bool act = m.sl < m.S && s < nseq;
+      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * sst : 0, 1, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
@@ -149,6 +157,20 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
         row[k] = post_r2c(A, B, twr[k]);
         if (k2 != k) row[k2] = post_r2c(B, A, twr[k2]);
       }
+    }
+    __syncthreads();
+  } else {
+    // odd length: Z = X + i Y of rows 2j, 2j+1 -> X[k] = (Z[k] + conj Z[N-k]) / 2
+    // into slot 2j, Y[k] = (Z[k] - conj Z[N-k]) / 2i into slot 2j+1, k <= N/2
+    // (position k < nlh is read and written by its own thread only; N-k >= nlh)
+    const int N = pp.m, h = pp.nlh, tot = ((nrows + 1) / 2) * h;
+    for (int e = tid; e < tot; e += nthr) {
+      const int j = (int)fdiv((uint32_t)e, pp.nlhdiv);
+      const int k = e - j * h;
+      cpx<T>* z = lds + 2 * j * pp.rsl;
+      const cpx<T> A = z[k], B = k ? z[N - k] : A;
+      z[k] = cpx<T>{T(0.5) * (A.x + B.x), T(0.5) * (A.y - B.y)};
+      if (2 * j + 1 < nrows) z[pp.rsl + k] = cpx<T>{T(0.5) * (A.y + B.y), T(0.5) * (B.x - A.x)};
     }
     __syncthreads();
   }
@@ -250,44 +272,61 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
       }
     }
   } else {
-    // odd length: Hermitian completion X[n] = conj X[nl - n]
-    const int nl = pp.nl, nh = pp.nlh, ext = nl - nh, tot = nrows * ext;
+    // odd length: the half spectra X, Y of rows 2j, 2j+1 become the full
+    // spectrum of x + i y in slot 2j: Z[k] = X[k] + i Y[k] and
+    // Z[N-k] = conj X[k] + i conj Y[k] (k >= 1); the k = 0 entries are real
+    const int N = pp.nl, h = pp.nlh, tot = ((nrows + 1) / 2) * h;
     for (int e = tid; e < tot; e += nthr) {
-      const int qq = e / ext;
-      const int n = nh + (e - qq * ext);
-      cpx<T>* row = lds + qq * pp.rsl;
-      row[n] = conj(row[nl - n]);
+      const int j = (int)fdiv((uint32_t)e, pp.nlhdiv);
+      const int k = e - j * h;
+      cpx<T>* z = lds + 2 * j * pp.rsl;
+      const cpx<T> X = z[k];
+      const cpx<T> Y = 2 * j + 1 < nrows ? z[pp.rsl + k] : cpx<T>{T(0), T(0)};
+      if (k) {
+        z[k] = cpx<T>{X.x - Y.y, X.y + Y.x};
+        z[N - k] = cpx<T>{X.x + Y.y, Y.x - X.y};
+      } else {
+        z[0] = cpx<T>{X.x, Y.x};
+      }
     }
   }
   __syncthreads();
 
-  // 4. inverse row FFTs
+  // 4. inverse row FFTs (odd lengths: one per pair of rows)
   {
+    const int nseq = pp.even ? nrows : (nrows + 1) / 2;
+    const int sst = pp.even ? pp.rsl : 2 * pp.rsl;
     const SeqThreads m = seq_threads(tid, nthr, pp.row, false);
-    for (int r0 = 0; r0 < nrows; r0 += m.S) {
+    for (int r0 = 0; r0 < nseq; r0 += m.S) {
       const int s = r0 + m.sl;
-      const bool act = m.sl < m.S && s < nrows;
-      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * pp.rsl : 0, 1, m.lane, act, m.wave);
+      const bool act = m.sl < m.S && s < nseq;
+      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * sst : 0, 1, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
 
-  // 5. store real rows
+  // 5. store real rows (odd lengths: rows 2j and 2j+1 from the real and
+  //    imaginary parts of slot 2j)
   {
-    const int m = pp.m, tot = nrows * m, rsl = pp.rsl;
-    auto get = [&](int e) {
-      const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-      return lds[qq * rsl + (e - qq * m)];
-    };
+    const int m = pp.m, rsl = pp.rsl;
     if (pp.even) {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, get, [&](int e, cpx<T> v) {
+      staged_copy<kLoads<T>, cpx<T>>(nrows * m, tid, nthr, [&](int e) {
+        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
+        return lds[qq * rsl + (e - qq * m)];
+      }, [&](int e, cpx<T> v) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
         reinterpret_cast<cpx<T>*>(out + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m] = v;
       });
     } else {
-      staged_copy<kLoads<T>, cpx<T>>(tot, tid, nthr, get, [&](int e, cpx<T> v) {
-        const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        out[(row0 + (int64_t)qq * pst) * pp.nl + (e - qq * m)] = v.x;
+      staged_copy<kLoads<T>, cpx<T>>(((nrows + 1) / 2) * m, tid, nthr, [&](int e) {
+        const int j = (int)fdiv((uint32_t)e, pp.mdiv);
+        return lds[2 * j * rsl + (e - j * m)];
+      }, [&](int e, cpx<T> v) {
+        const int j = (int)fdiv((uint32_t)e, pp.mdiv);
+        const int n = e - j * m;
+        const gptr<T> ra = out + (row0 + (int64_t)(2 * j) * pst) * pp.nl;
+        ra[n] = v.x;
+        if (2 * j + 1 < nrows) ra[(int64_t)pst * pp.nl + n] = v.y;
       });
     }
   }
