@@ -37,6 +37,21 @@ __device__ __forceinline__ void staged_copy(int tot, int tid, int nthr, Get get,
   }
 }
 
+// The field is read once (forward) and written once (inverse): non-temporal
+// accesses leave the last-level cache to the half-spectrum workspace, which
+// the next sweep reads from there.
+template <typename T>
+__device__ __forceinline__ cpx<T> nt_load_c(gptr<const cpx<T>> p) {
+  const gptr<const T> q = (gptr<const T>)p;
+  return cpx<T>{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1)};
+}
+template <typename T>
+__device__ __forceinline__ void nt_store_c(gptr<cpx<T>> p, cpx<T> v) {
+  const gptr<T> q = (gptr<T>)p;
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> post_r2c(cpx<T> A, cpx<T> B, cpx<T> w) {
   // X[k] = (A + conj B)/2 - i w (A - conj B)/2, A = Z[k], B = Z[m-k]
@@ -108,7 +123,7 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
     if (pp.even) {
       staged_copy<kLoads<T>, cpx<T>>(nrows * m, tid, nthr, [&](int e) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        return ((gptr<const cpx<T>>)(u + (row0 + (int64_t)qq * pst) * pp.nl))[e - qq * m];
+        return nt_load_c<T>((gptr<const cpx<T>>)(u + (row0 + (int64_t)qq * pst) * pp.nl) + (e - qq * m));
       }, [&](int e, cpx<T> v) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
         lds[qq * rsl + (e - qq * m)] = v;
@@ -118,7 +133,8 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
         const int j = (int)fdiv((uint32_t)e, pp.mdiv);
         const int n = e - j * m;
         const gptr<const T> ra = u + (row0 + (int64_t)(2 * j) * pst) * pp.nl;
-        return cpx<T>{ra[n], 2 * j + 1 < nrows ? ra[(int64_t)pst * pp.nl + n] : T(0)};
+        return cpx<T>{__builtin_nontemporal_load(ra + n),
+                      2 * j + 1 < nrows ? __builtin_nontemporal_load(ra + (int64_t)pst * pp.nl + n) : T(0)};
       }, [&](int e, cpx<T> v) {
         const int j = (int)fdiv((uint32_t)e, pp.mdiv);
         lds[2 * j * rsl + (e - j * m)] = v;
@@ -315,7 +331,7 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
         return lds[qq * rsl + (e - qq * m)];
       }, [&](int e, cpx<T> v) {
         const int qq = (int)fdiv((uint32_t)e, pp.mdiv);
-        reinterpret_cast<cpx<T>*>(out + (row0 + (int64_t)qq * pst) * pp.nl)[e - qq * m] = v;
+        nt_store_c<T>((gptr<cpx<T>>)(out + (row0 + (int64_t)qq * pst) * pp.nl) + (e - qq * m), v);
       });
     } else {
       staged_copy<kLoads<T>, cpx<T>>(((nrows + 1) / 2) * m, tid, nthr, [&](int e) {
@@ -325,8 +341,8 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
         const int j = (int)fdiv((uint32_t)e, pp.mdiv);
         const int n = e - j * m;
         const gptr<T> ra = out + (row0 + (int64_t)(2 * j) * pst) * pp.nl;
-        ra[n] = v.x;
-        if (2 * j + 1 < nrows) ra[(int64_t)pst * pp.nl + n] = v.y;
+        __builtin_nontemporal_store(v.x, ra + n);
+        if (2 * j + 1 < nrows) __builtin_nontemporal_store(v.y, ra + (int64_t)pst * pp.nl + n);
       });
     }
   }
