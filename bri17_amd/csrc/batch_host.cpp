@@ -108,7 +108,7 @@ int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* 
     int rc = b17_ensure_ws(op);
     if (rc) return rc;
     insts[i] = b17_batch_inst(*op->plan, op->mu, op->nu, items[i].mode, items[i].in, items[i].out,
-                              op->ws, lds);
+                              op->ws, threads, lds);
   }
   return b17_batch_launch(b, dtype, insts, threads, lds, (hipStream_t)stream);
 }
@@ -136,11 +136,11 @@ void b17_batch_shape(const std::vector<const b17::Plan*>& plans, int& threads, s
 }
 
 b17::BatchInst b17_batch_inst(const b17::Plan& pl, double mu, double nu, int mode, const void* in,
-                              void* out, void* ws, size_t lds) {
+                              void* out, void* ws, int threads, size_t lds) {
   b17::BatchInst I;
   std::memset(&I, 0, sizeof(I));
   I.pc = b17::pencil_for(pl, mu, nu, mode);
-  (void)b17::batch_geometry(pl, lds, I.pp, I.na, I.pc);  // fits: lds >= batch_minimum
+  (void)b17::batch_geometry(pl, lds, threads, I.pp, I.na, I.pc);  // fits: lds >= batch_minimum
   I.pc.xcd_remap = 0;
   I.in = in;
   I.out = out;

@@ -46,7 +46,7 @@ void b17_batch_shape(const std::vector<const b17::Plan*>& plans, int& threads, s
 // descriptor of one instance of a persistent batched launch whose
 // workgroups have `lds` bytes of LDS (from b17_batch_shape)
 b17::BatchInst b17_batch_inst(const b17::Plan& pl, double mu, double nu, int mode, const void* in,
-                              void* out, void* ws, size_t lds);
+                              void* out, void* ws, int threads, size_t lds);
 // upload an instance table and launch the persistent kernel on `st` (waits
 // for the context's previous launch first); threads / lds are the maxima
 // over the instances' plans.  The caller serialises use of `b`.

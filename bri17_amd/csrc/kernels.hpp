@@ -71,6 +71,9 @@ struct PlaneParams {
   int has_col;   // 3D: do the Q-point middle-axis stage
   int rs;        // workspace row stride (complex elements)
   int nrows;     // 2D: total number of rows (components * N0); 3D unused
+  int g3;        // 3D, split 1: consecutive planes per work item (1 outside
+                 // the batch kernel)
+  int nplanes;   // 3D with g3 > 1: planes of the field (components * N0)
   FastDiv mdiv;  // division by m
   FastDiv hdiv;  // division by m/2+1 (pairs per row)
   FastDiv nlhdiv;

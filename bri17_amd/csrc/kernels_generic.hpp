@@ -80,7 +80,7 @@ __device__ __forceinline__ cpx<T> pre_c2r(cpx<T> A, cpx<T> B, cpx<T> w) {
 // pass would otherwise wait out.
 template <typename T>
 __device__ __forceinline__ cpx<T>* plane_tables(cpx<T>* lds, const PlaneParams& pp) {
-  return lds + pp.q * pp.rsl;
+  return lds + pp.g3 * pp.q * pp.rsl;
 }
 
 template <typename T>
@@ -101,12 +101,18 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
                                                gptr<cpx<T>> __restrict__ ws, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int plane = blk / pp.p;
-  const int p = blk - plane * pp.p;
+  const int grp = blk / pp.p;
+  const int p = blk - grp * pp.p;
+  // 3D items of the batch kernel may hold g3 consecutive planes (split 1:
+  // their rows are consecutive in memory and in LDS); elsewhere g3 = 1
+  const int plane = grp * pp.g3;
   int nrows = pp.q;
   if (!pp.has_col) {
     const int rem = pp.nrows - plane * pp.n1;
     nrows = rem < pp.q ? rem : pp.q;
+  } else if (pp.g3 > 1) {
+    const int np = pp.nplanes - plane;
+    nrows = pp.q * (np < pp.g3 ? np : pp.g3);
   }
   const int64_t row0 = (int64_t)plane * pp.n1 + p;
   const int pst = pp.p;
@@ -195,10 +201,13 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
   //    twiddle w_N1^(p ka) is applied on the way out
   if (pp.has_col) {
     const SeqThreads m = seq_threads(tid, nthr, pp.col, true);
-    for (int c0 = 0; c0 < pp.nlh; c0 += m.S) {
+    const int ncol = (pp.g3 > 1 ? nrows / pp.q : 1) * pp.nlh;  // columns of every plane
+    for (int c0 = 0; c0 < ncol; c0 += m.S) {
       const int s = c0 + m.sl;
-      const bool act = m.sl < m.S && s < pp.nlh;
-      lds_fft<T, false, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, m.lane, act, m.wave);
+      const bool act = m.sl < m.S && s < ncol;
+      const int g = act ? (int)fdiv((uint32_t)s, pp.nlhdiv) : 0;
+      const int base = act ? g * pp.q * pp.rsl + (s - g * pp.nlh) : 0;
+      lds_fft<T, false, kMaxE>(lds, twc, pp.col, base, pp.rsl, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
@@ -227,12 +236,18 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
                                                int blk, cpx<T>* lds) {
   const gptr<const cpx<T>> twc_g = (gptr<const cpx<T>>)pp.tw_col;
   const int tid = threadIdx.x, nthr = blockDim.x;
-  const int plane = blk / pp.p;
-  const int p = blk - plane * pp.p;
+  const int grp = blk / pp.p;
+  const int p = blk - grp * pp.p;
+  // 3D items of the batch kernel may hold g3 consecutive planes (split 1:
+  // their rows are consecutive in memory and in LDS); elsewhere g3 = 1
+  const int plane = grp * pp.g3;
   int nrows = pp.q;
   if (!pp.has_col) {
     const int rem = pp.nrows - plane * pp.n1;
     nrows = rem < pp.q ? rem : pp.q;
+  } else if (pp.g3 > 1) {
+    const int np = pp.nplanes - plane;
+    nrows = pp.q * (np < pp.g3 ? np : pp.g3);
   }
   const int64_t row0 = (int64_t)plane * pp.n1 + p;
   const int pst = pp.p;
@@ -260,10 +275,13 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
   // 2. 3D: inverse Q-point stage of the middle axis
   if (pp.has_col) {
     const SeqThreads m = seq_threads(tid, nthr, pp.col, true);
-    for (int c0 = 0; c0 < pp.nlh; c0 += m.S) {
+    const int ncol = (pp.g3 > 1 ? nrows / pp.q : 1) * pp.nlh;  // columns of every plane
+    for (int c0 = 0; c0 < ncol; c0 += m.S) {
       const int s = c0 + m.sl;
-      const bool act = m.sl < m.S && s < pp.nlh;
-      lds_fft<T, true, kMaxE>(lds, twc, pp.col, act ? s : 0, pp.rsl, m.lane, act, m.wave);
+      const bool act = m.sl < m.S && s < ncol;
+      const int g = act ? (int)fdiv((uint32_t)s, pp.nlhdiv) : 0;
+      const int base = act ? g * pp.q * pp.rsl + (s - g * pp.nlh) : 0;
+      lds_fft<T, true, kMaxE>(lds, twc, pp.col, base, pp.rsl, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }

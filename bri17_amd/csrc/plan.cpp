@@ -217,6 +217,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   PlaneParams& pp = pl->plane;
   std::memset(&pp, 0, sizeof(pp));
   pp.dim = dim;
+  pp.g3 = 1;
   pp.nl = nl;
   pp.nlh = nlh;
   pp.even = even;
@@ -362,7 +363,7 @@ void batch_minimum(const Plan& pl, size_t& lds, int& threads) {
   threads = round_up(t, 64);
 }
 
-bool batch_geometry(const Plan& pl, size_t lds, PlaneParams& pp, int& plane_blocks,
+bool batch_geometry(const Plan& pl, size_t lds, int threads, PlaneParams& pp, int& plane_blocks,
                     PencilParams& pc) {
   const size_t csz = 2 * pl.esz;
   pp = pl.plane;
@@ -377,6 +378,21 @@ bool batch_geometry(const Plan& pl, size_t lds, PlaneParams& pp, int& plane_bloc
     plane_blocks = (pp.nrows + G - 1) / G;
   } else if ((size_t)pp.q * pp.rsl * csz + plane_tab > lds) {
     return false;
+  } else if (pp.p == 1 && (pp.even || pp.q % 2 == 0)) {
+    // Several consecutive planes per item while the workgroup has threads
+    // for all their row and column sequences in one round and the LDS holds
+    // them.  (Odd rows are transformed in pairs: an item must not pair the
+    // last row of one plane with the first of the next, hence even Q.)
+    const int planes = pl.plane_blocks;  // components * N0 with split 1
+    const int rseq = pp.even ? pp.q : pp.q / 2;
+    const int per = std::max(rseq * pp.row.tpt, pp.nlh * pp.col.tpt);
+    int g = std::min<int>(planes, (int)((lds - plane_tab) / ((size_t)pp.q * pp.rsl * csz)));
+    g = std::min(g, std::max(1, threads / per));
+    if (g > 1) {
+      pp.g3 = g;
+      pp.nplanes = planes;
+      plane_blocks = (planes + g - 1) / g;
+    }
   }
   const size_t per_w = (size_t)pl.dim * pc.n0 * pc.p * csz;
   const size_t ptab = (size_t)pencil_table_entries(pc) * csz;

@@ -240,7 +240,7 @@ int launch_wave(bri17_server* s, Slot& sl) {
     insts.reserve(sl.reqs.size());
     for (auto& q : sl.reqs)
       insts.push_back(b17_batch_inst(*q->plan, q->r.mu, q->r.nu, q->r.mode, in_of(*q), out_of(*q),
-                                     q->ws, lds));
+                                     q->ws, threads, lds));
     *sl.h_status = 0;
     int spin;
     {
