@@ -70,6 +70,7 @@ struct Req {
   void* ws = nullptr;
   void* d_in = nullptr;
   void* d_out = nullptr;
+  void* h_stage = nullptr;  // host-memory requests: pinned staging (in, then out)
 };
 
 struct Slot {
@@ -104,6 +105,7 @@ struct bri17_server {
   // ---- worker-owned
   std::vector<Slot> slots;
   std::map<size_t, std::vector<void*>> idle;
+  std::map<size_t, std::vector<void*>> idle_host;  // pinned staging buffers
   std::thread worker;
 };
 
@@ -126,6 +128,10 @@ void evict_one(bri17_server* s) {
 }
 
 void evict_all(bri17_server* s) {
+  for (auto& kv : s->idle_host) {
+    for (void* p : kv.second) (void)hipHostFree(p);
+    kv.second.clear();
+  }
   for (auto& kv : s->idle) {
     for (void* p : kv.second) (void)hipFree(p);
     std::lock_guard<std::mutex> g(s->mtx);
@@ -218,13 +224,34 @@ void unadmit(bri17_server* s, Req& q) {
   s->admitted -= q.footprint;
 }
 
+// Host-memory requests are staged through a pinned buffer: the worker copies
+// the input into it, and the copies to and from the device are then
+// asynchronous (a copy from or to pageable memory would hold the worker
+// until the wave's kernels finish); the result is copied out when the wave
+// retires.
+void release_host(bri17_server* s, Req& q) {
+  if (q.h_stage) s->idle_host[q.stage_cls].push_back(q.h_stage);
+  q.h_stage = nullptr;
+}
+
 // ---------------------------------------------------------------- waves
 int launch_wave(bri17_server* s, Slot& sl) {
   hipStream_t st = sl.stream;
-  for (auto& q : sl.reqs)
-    if (q->r.memory == BRI17_MEM_HOST &&
-        hipMemcpyAsync(q->d_in, q->r.in, q->field_bytes, hipMemcpyHostToDevice, st) != hipSuccess)
+  for (auto& q : sl.reqs) {
+    if (q->r.memory != BRI17_MEM_HOST) continue;
+    auto& pool = s->idle_host[q->stage_cls];
+    if (!pool.empty()) {
+      q->h_stage = pool.back();
+      pool.pop_back();
+    } else if (hipHostMalloc(&q->h_stage, q->stage_cls, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      q->h_stage = nullptr;
+      return BRI17_ERR_NOMEM;
+    }
+    std::memcpy(q->h_stage, q->r.in, q->field_bytes);
+    if (hipMemcpyAsync(q->d_in, q->h_stage, q->field_bytes, hipMemcpyHostToDevice, st) != hipSuccess)
       return BRI17_ERR_HIP;
+  }
   auto in_of = [](const Req& q) { return q.r.memory == BRI17_MEM_HOST ? q.d_in : q.r.in; };
   auto out_of = [](const Req& q) { return q.r.memory == BRI17_MEM_HOST ? q.d_out : q.r.out; };
   // A wave of one batchable request runs the plan's own three kernels: the
@@ -260,7 +287,7 @@ int launch_wave(bri17_server* s, Slot& sl) {
   }
   for (auto& q : sl.reqs)
     if (q->r.memory == BRI17_MEM_HOST &&
-        hipMemcpyAsync(q->r.out, q->d_out, q->field_bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
+        hipMemcpyAsync(q->h_stage, q->d_out, q->field_bytes, hipMemcpyDeviceToHost, st) != hipSuccess)
       return BRI17_ERR_HIP;
   return hipEventRecord(sl.done, st) == hipSuccess ? BRI17_OK : BRI17_ERR_HIP;
 }
@@ -269,7 +296,13 @@ void finish_wave(bri17_server* s, Slot& sl, int code) {
   const Clock::time_point t = Clock::now();
   if (!code && sl.batched && sl.reqs.size() > 1 && *sl.h_status)
     code = BRI17_ERR_STATE;  // a dependency wait timed out
-  for (auto& q : sl.reqs) unadmit(s, *q);
+  for (auto& q : sl.reqs) {
+    if (q->r.memory == BRI17_MEM_HOST && q->h_stage) {
+      if (!code) std::memcpy(q->r.out, q->h_stage, q->field_bytes);
+      release_host(s, *q);
+    }
+    unadmit(s, *q);
+  }
   {
     std::lock_guard<std::mutex> g(s->mtx);
     for (auto& q : sl.reqs) {
