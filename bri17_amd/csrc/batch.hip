@@ -7,7 +7,11 @@
 // inverse) and inverse plane blocks -- to one queue, ordered so that an item
 // only ever depends on items that precede it.  Workgroups take items from an
 // atomic head counter; a pencil item waits until all plane blocks of its
-// instance are done, an inverse plane item until all its pencil tiles are.
+// instance are done, an inverse plane item until every pencil tile of the
+// launch is.  (Waiting per instance let inverse items start a little
+// earlier, but each of those waits ended in an acquire that invalidates an
+// XCD's L2 while the last pencils still run: 5% slower on a 64-instance
+// launch.)
 //
 // Publication.  Making a workgroup's workspace writes visible to the other
 // XCDs takes a device-scope release, which writes back this XCD's whole L2;
@@ -19,7 +23,9 @@
 // counters, two global counters count the finished forward items and
 // pencil tiles of all instances; once a workgroup has seen one of them
 // complete and acquired after that, its later items of the next kind skip
-// the wait and the acquire (which invalidates this XCD's L2).
+// the wait and the acquire (which invalidates this XCD's L2).  Inverse
+// items wait on the global pencil counter only, so a workgroup acquires for
+// them once.
 //
 // Progress.  A workgroup never waits while holding unpublished items, and
 // forward items never wait, so the lowest unpublished item is always
@@ -97,8 +103,10 @@ __global__ void __launch_bounds__(1024)
               total) {
             seen = true;
           } else {
-            const int need = type == 1 ? I->na : I->nb;
-            int* cnt = done + 2 * inst + (type - 1);
+            // pencils: this instance's forward items; inverse items: every
+            // pencil tile of the launch
+            const int need = type == 1 ? I->na : total;
+            int* cnt = type == 1 ? done + 2 * inst : all + 1;
             int n = 0;
             while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
               if (++n > spin_limit) {
@@ -107,6 +115,7 @@ __global__ void __launch_bounds__(1024)
               }
               __builtin_amdgcn_s_sleep(4);
             }
+            if (type == 2) seen = true;
           }
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
