@@ -156,12 +156,25 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
   if (rc) return rc;
   std::vector<int2> work;
   // queue: every forward plane block, then every pencil tile, then every
-  // inverse plane block -- an item depends only on items before it
-  for (int type = 0; type < 3; type++)
-    for (int i = 0; i < n; i++) {
+  // inverse plane block -- an item depends only on items before it.  Within
+  // each kind, instances with the largest items come first, so that the
+  // end of each phase is filled with small items.
+  auto item_size = [&](int i, int type) -> int64_t {
+    const b17::BatchInst& I = insts[i];
+    if (type == 1) return (int64_t)I.pc.dim * I.pc.n0 * I.pc.p * I.pc.w;
+    const int rows = I.pp.has_col ? I.pp.q * I.pp.g3 : I.pp.q;
+    return (int64_t)rows * I.pp.nl;
+  };
+  std::vector<int> order((size_t)n);
+  for (int type = 0; type < 3; type++) {
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return item_size(a, type) > item_size(b, type); });
+    for (int i : order) {
       const int cnt = type == 1 ? insts[i].nb : insts[i].na;
       for (int k = 0; k < cnt; k++) work.push_back(make_int2(i | (type << 16), k));
     }
+  }
   const int nitems = (int)work.size();
   const int nctl = 4 + 2 * n;
   int n_fwd = 0, n_pen = 0;
