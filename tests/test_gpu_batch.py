@@ -105,34 +105,39 @@ def test_batch_in_fp64_matches_oracle(device, oracle):
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_batch_multi_plane_items_and_odd_rows(device, oracle, dtype):
-    """Plane items that hold several consecutive planes (split-1 plans) and
-    odd rows transformed in pairs: odd last axes with an even middle axis
-    (pairs inside a plane, several planes per item), with an odd middle
-    axis (one plane per item, so that no pair straddles two planes), an even
-    last axis, and a split-3 plan with odd rows.  Each instance equals its
-    solo runtime-size result bitwise and the fp64 oracle within tolerance."""
+    """Plane items that hold several consecutive planes and odd rows
+    transformed in pairs: odd last axes with an even middle axis (pairs
+    inside a plane, several planes per item), with an odd middle axis (one
+    plane per item, so that no pair straddles two planes), an even last
+    axis, a split-3 plan with odd rows and an odd Q, and split-2 plans (one
+    plane per item, rows strided in memory).
+    Each instance equals its solo runtime-size result bitwise and the fp64
+    oracle within tolerance."""
     import torch
     from bri17_amd import Batch, StiffnessMatrix
-    spec = [((12, 30, 27), 1.3, 0.2, "apply"), ((10, 27, 27), 0.8, 0.3, "solve"),
-            ((8, 16, 45), 1.1, 0.1, "apply"), ((9, 24, 32), 1.7, 0.35, "solve"),
-            ((6, 14, 81), 0.9, 0.25, "apply"), ((5, 81, 27), 1.2, 0.15, "apply")]
+    # (shape, mu, nu, mode, middle-axis split)
+    spec = [((12, 30, 27), 1.3, 0.2, "apply", 1), ((10, 27, 27), 0.8, 0.3, "solve", 1),
+            ((8, 16, 45), 1.1, 0.1, "apply", 1), ((9, 24, 32), 1.7, 0.35, "solve", 1),
+            ((6, 14, 81), 0.9, 0.25, "apply", 1), ((5, 81, 27), 1.2, 0.15, "apply", 3),
+            ((6, 16, 27), 1.4, 0.3, "solve", 2), ((7, 24, 32), 0.6, 0.05, "apply", 2)]
     g = torch.Generator(device=device).manual_seed(31)
     xs = []
-    for shape, _, _, mode in spec:
+    for shape, _, _, mode, _ in spec:
         x = torch.rand((3,) + shape, generator=g, device=device, dtype=getattr(torch, dtype)) * 2 - 1
         if mode == "solve":
             x -= x.mean(dim=(1, 2, 3), keepdim=True)
         xs.append(x)
     ops = [StiffnessMatrix(s, (1.0, 1.2, 0.9), mu, nu, dtype,
-                           tuning=dict(force_generic=1, plane_split=3 if s[1] == 81 else 0))
-           for s, mu, nu, _ in spec]
+                           tuning=dict(force_generic=1, plane_split=sp))
+           for s, mu, nu, _, sp in spec]
     outs = [torch.empty_like(x) for x in xs]
     b = Batch()
-    b.run([(op, m, x, o) for op, (s, mu, nu, m), x, o in zip(ops, spec, xs, outs)])
+    b.run([(op, m, x, o) for op, (s, mu, nu, m, _), x, o in zip(ops, spec, xs, outs)])
     assert b.wait() == 0
     print("batch geometry", b.info(), "splits", [op.describe()["plane"]["split"] for op in ops], flush=True)
+    assert [op.describe()["plane"]["split"] for op in ops] == [sp for *_, sp in spec]
     tol = 1e-12 if dtype == "float64" else 1e-5
-    for (s, mu, nu, m), op, x, o in zip(spec, ops, xs, outs):
+    for (s, mu, nu, m, _), op, x, o in zip(spec, ops, xs, outs):
         r = op.apply(x) if m == "apply" else op.solve(x)
         torch.cuda.synchronize()
         assert torch.equal(r, o), (s, m)
