@@ -396,6 +396,9 @@ bool start_wave(bri17_server* s, Slot& sl) {
       back_from = 1;
       s->cv_done.notify_all();
     }
+    // (a cancellation may have shortened the queue since the requests were
+    // taken out)
+    at = std::min(at, s->queue.size());
     s->queue.insert(s->queue.begin() + (std::ptrdiff_t)at, take.begin() + (std::ptrdiff_t)back_from,
                     take.end());
     take.resize(n_ok);
