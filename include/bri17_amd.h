@@ -212,7 +212,10 @@ typedef struct bri17_batch_item {
 
 int bri17_batch_create(bri17_batch** out);
 int bri17_batch_destroy(bri17_batch* b);
-/* Enqueue the batch on `stream` (asynchronous). */
+/* Enqueue the batch on `stream` (asynchronous).  A run whose items equal
+ * the previous run's on the same stream (operators, modes, buffers) reuses
+ * the device work queue and is enqueued without waiting; any other run first
+ * waits for the previous one, then uploads a new queue. */
 int bri17_batch_run(bri17_batch* b, const bri17_batch_item* items, int n, void* stream);
 /* Wait for the last run; *dependency_timeouts != 0 if a work item gave up
  * waiting for its producers (spin limit reached: results are invalid). */
