@@ -310,6 +310,18 @@ def batched(dev, steps, warmup, seed=4):
     torch.cuda.synchronize()
     t = (time.perf_counter() - t0) / steps
     assert b.wait() == 0
+    # the same with a new queue every launch (outputs alternate between two
+    # buffer sets, so no launch repeats the previous one's items)
+    outs2 = [torch.empty_like(x) for x in xs]
+    items2 = [(op, m, x, o) for op, (_, _, _, m), x, o in zip(ops, spec, xs, outs2)]
+    b.run(items2)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for k in range(steps):
+        b.run(items if k % 2 == 0 else items2)
+    torch.cuda.synchronize()
+    t_new = (time.perf_counter() - t2) / steps
+    assert b.wait() == 0
     nodes = sum(int(np.prod(sh)) for sh, *_ in spec)
     b_alg = sum(2 * len(sh) * int(np.prod(sh)) * 4 for sh, *_ in spec)
     # the same 64 instances one operator call after another, for comparison
@@ -325,6 +337,9 @@ def batched(dev, steps, warmup, seed=4):
     torch.cuda.synchronize()
     t_seq = (time.perf_counter() - t1) / steps
     return {"instances": 64, "nodes": nodes, "ms_per_launch": 1e3 * t, "instances_per_s": 64 / t,
+            "queue": "the same items every step: each launch after the first reuses the device work "
+                     "queue and is enqueued without waiting for the previous one",
+            "ms_per_launch_new_queue": 1e3 * t_new,
             "achieved_GBs": b_alg / t / 1e9, "algorithmic_bytes": b_alg,
             "ms_sequential_calls": 1e3 * t_seq, "speedup_vs_sequential": t_seq / t, "launch": b.info()}
 
