@@ -396,7 +396,11 @@ bool batch_geometry(const Plan& pl, size_t lds, int threads, PlaneParams& pp, in
   }
   const size_t per_w = (size_t)pl.dim * pc.n0 * pc.p * csz;
   const size_t ptab = (size_t)pencil_table_entries(pc) * csz;
-  int W = pc.w;
+  // The widest tile that fits, whatever the plan's own width: a column's
+  // arithmetic does not depend on the tile it is in, and the batch's
+  // workgroups have threads for wide tiles.
+  int W = 1;
+  while (W < 64 && W < pc.f) W *= 2;
   while (W > 1 && per_w * W + ptab > lds) W /= 2;
   if (per_w * W + ptab > lds) return false;
   pc.w = W;

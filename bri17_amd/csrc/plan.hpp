@@ -71,10 +71,10 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype,
 
 // Work items of a plan inside the persistent batch kernel: the same
 // arithmetic as the plan (radix orders, middle-axis split), with the 2D
-// plane row group and the pencil tile width narrowed so that every item fits
-// `lds` bytes, and 3D plane items of split 1 grouped by consecutive planes
-// up to what `threads` transform in one round.  false: a single row or
-// column does not fit.
+// plane row group narrowed and the pencil tile the widest (up to 64
+// columns) so that every item fits `lds` bytes, and 3D plane items of split
+// 1 grouped by consecutive planes up to what `threads` transform in one
+// round.  false: a single row or column does not fit.
 bool batch_geometry(const Plan& pl, size_t lds, int threads, PlaneParams& pp, int& plane_blocks,
                     PencilParams& pc);
 // Least LDS (bytes) and threads with which every item of the plan can run.
