@@ -371,7 +371,18 @@ bool batch_geometry(const Plan& pl, size_t lds, int threads, PlaneParams& pp, in
   const size_t plane_tab = (size_t)plane_table_entries(pp) * csz;
   if (plane_tab >= lds) return false;
   if (!pp.has_col) {
-    int G = std::min<int>(pp.q, (int)((lds - plane_tab) / ((size_t)pp.rsl * csz)));
+    // 2D: as many rows per item as fit the LDS and the workgroup transforms
+    // in one round.  Odd rows are transformed in pairs (2j, 2j+1) counted
+    // from the item's first row, so the row group stays even -- the pairs of
+    // the plan's own even row group -- and a plan with an odd row group keeps
+    // it.
+    const int fit = (int)((lds - plane_tab) / ((size_t)pp.rsl * csz));
+    int G = std::min(fit, pp.q);
+    if (pp.even || pp.q % 2 == 0) {
+      const int per_row = pp.even ? pp.row.tpt : (pp.row.tpt + 1) / 2;
+      G = std::min({fit, pp.nrows, std::max(pp.q, threads / std::max(1, per_row))});
+      if (!pp.even && G > 1) G &= ~1;
+    }
     if (G < 1) return false;
     pp.q = G;
     pp.n1 = G;
