@@ -581,6 +581,11 @@ def main():
         line["batched_64"] = batched(dev, max(5, K // 10), 3)
     if not args.no_serve:
         line["serving"] = serving(dev, n_requests=args.serve_requests)
+        # the same workload at a fixed arrival rate: comparable across builds
+        # whose capacity (and hence the rate of the line above) differs
+        fixed = serving(dev, n_requests=args.serve_requests, rate=5000.0)
+        line["serving_fixed_rate"] = {k: fixed[k] for k in (
+            "arrival_rate_hz", "throughput_rps", "p50_ms", "p99_ms", "mean_ms", "max_ms", "completed")}
     if not args.no_cpu:
         # every CPU this process may use, within the host's OpenMP thread
         # setting (the CPU share of one GPU on a shared host)
