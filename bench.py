@@ -101,8 +101,9 @@ def _traffic(name):
 
 def _valu(name, kernel_ms):
     """VALU issue time of kernel `name` per launch, from the committed SQ
-    counter summary (SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x 2.4 GHz)), as a
-    fraction of its measured duration."""
+    counter summary (SQ_INSTS_VALU x the mean issue interval of the kernel's
+    instruction mix / 1024 SIMDs; the intervals are measured by
+    bench_issue_rate.hip), as a fraction of its measured duration."""
     p = os.path.join(ROOT, "profiles", "sq_256cube_f32.json")
     try:
         with open(p) as f:
@@ -557,7 +558,8 @@ def main():
                    "grid": list(shape), "components": 3, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": _traffic(dom),
-                     "limiter": "valu issue + latency (one workgroup per CU)",
+                     "limiter": "latency: LDS, memory and barriers at one workgroup per CU "
+                                "(VALU issue 34-43%, HBM 55-70% of each kernel's time)",
                      "valu": _valu(dom, per_kernel[dom]) if n == 256 else None},
         "per_kernel_ms": per_kernel,
         "apply_roofline": {"algorithmic_bytes": b_alg, "achieved_GBs": b_alg / (ms_step * 1e-3) / 1e9,
