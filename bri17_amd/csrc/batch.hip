@@ -41,7 +41,7 @@ namespace b17 {
 // finished items a workgroup publishes with one release
 constexpr int kPend = 8;
 
-template <typename T>
+template <typename T, bool PR>
 __global__ void __launch_bounds__(1024)
     k_batch(const BatchInst* __restrict__ insts, const int2* __restrict__ items, int nitems,
             int n_fwd, int n_pen, int* __restrict__ ctl, int spin_limit) {
@@ -123,16 +123,16 @@ __global__ void __launch_bounds__(1024)
       __syncthreads();
     }
     if (type == 0) {
-      plane_fwd_body<T>((gptr<const T>)I->in, (gptr<cpx<T>>)I->ws, I->pp, idx, lds);
+      plane_fwd_body<T, PR>((gptr<const T>)I->in, (gptr<cpx<T>>)I->ws, I->pp, idx, lds);
     } else if (type == 2) {
-      plane_inv_body<T>((gptr<const cpx<T>>)I->ws, (gptr<T>)I->out, I->pp, idx, lds);
+      plane_inv_body<T, PR>((gptr<const cpx<T>>)I->ws, (gptr<T>)I->out, I->pp, idx, lds);
     } else {
       gptr<cpx<T>> ws = (gptr<cpx<T>>)I->ws;
       switch (I->pc.dim * 2 + I->pc.mode) {
-        case 4: pencil_body<T, 2, 0>(ws, I->pc, idx, lds); break;
-        case 5: pencil_body<T, 2, 1>(ws, I->pc, idx, lds); break;
-        case 6: pencil_body<T, 3, 0>(ws, I->pc, idx, lds); break;
-        default: pencil_body<T, 3, 1>(ws, I->pc, idx, lds); break;
+        case 4: pencil_body<T, 2, 0, PR>(ws, I->pc, idx, lds); break;
+        case 5: pencil_body<T, 2, 1, PR>(ws, I->pc, idx, lds); break;
+        case 6: pencil_body<T, 3, 0, PR>(ws, I->pc, idx, lds); break;
+        default: pencil_body<T, 3, 1, PR>(ws, I->pc, idx, lds); break;
       }
     }
     __syncthreads();  // the LDS is the next item's
@@ -152,14 +152,15 @@ __global__ void k_batch_reset(int* __restrict__ ctl, int n) {
   if (i < n) ctl[i] = 0;
 }
 
-hipError_t launch_batch(int dtype, const BatchInst* insts, const int2* items, int nitems,
-                        int n_fwd, int n_pen, int* ctl, int nctl, int threads, size_t lds,
-                        int spin_limit, hipStream_t st, int* grid_out) {
+hipError_t launch_batch(int dtype, bool primes, const BatchInst* insts, const int2* items,
+                        int nitems, int n_fwd, int n_pen, int* ctl, int nctl, int threads,
+                        size_t lds, int spin_limit, hipStream_t st, int* grid_out) {
   hipLaunchKernelGGL(k_batch_reset, dim3((nctl + 255) / 256), dim3(256), 0, st, ctl, nctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const void* fn = dtype == 0 ? reinterpret_cast<const void*>(k_batch<float>)
-                              : reinterpret_cast<const void*>(k_batch<double>);
+  auto kf = primes ? k_batch<float, true> : k_batch<float, false>;
+  auto kd = primes ? k_batch<double, true> : k_batch<double, false>;
+  const void* fn = dtype == 0 ? reinterpret_cast<const void*>(kf) : reinterpret_cast<const void*>(kd);
   if (lds > 65536) {
     e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -175,11 +176,11 @@ hipError_t launch_batch(int dtype, const BatchInst* insts, const int2* items, in
   if (grid < 1) grid = 1;
   if (grid_out) *grid_out = grid;
   if (dtype == 0)
-    hipLaunchKernelGGL(k_batch<float>, dim3(grid), dim3(threads), lds, st, insts, items, nitems,
-                       n_fwd, n_pen, ctl, spin_limit);
+    hipLaunchKernelGGL(kf, dim3(grid), dim3(threads), lds, st, insts, items, nitems, n_fwd, n_pen,
+                       ctl, spin_limit);
   else
-    hipLaunchKernelGGL(k_batch<double>, dim3(grid), dim3(threads), lds, st, insts, items, nitems,
-                       n_fwd, n_pen, ctl, spin_limit);
+    hipLaunchKernelGGL(kd, dim3(grid), dim3(threads), lds, st, insts, items, nitems, n_fwd, n_pen,
+                       ctl, spin_limit);
   return hipGetLastError();
 }
 

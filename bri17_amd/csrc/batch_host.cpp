@@ -25,6 +25,7 @@ struct bri17_batch {
   int spin_limit = 1 << 24;
   int last_nitems = 0, last_grid = 0, last_threads = 0, last_dtype = 0;
   int last_nfwd = 0, last_npen = 0, last_n = 0;
+  bool last_primes = false;  // the launch holds radix-11 / radix-13 transforms
   hipStream_t last_stream = nullptr;
   size_t last_lds = 0, last_off_items = 0;
   int graphs = 0;  // live graphs captured from this context (they replay its tables)
@@ -163,7 +164,8 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
       std::memcmp(b->h_buf, insts.data(), sizeof(b17::BatchInst) * n) == 0) {
     char* d = static_cast<char*>(b->d_buf);
     int grid = 0;
-    hipError_t e = b17::launch_batch(dtype, reinterpret_cast<const b17::BatchInst*>(d),
+    hipError_t e = b17::launch_batch(dtype, b->last_primes,
+                                     reinterpret_cast<const b17::BatchInst*>(d),
                                      reinterpret_cast<const int2*>(d + b->last_off_items),
                                      b->last_nitems, b->last_nfwd, b->last_npen, b->d_ctl, b->nctl,
                                      threads, lds, b->spin_limit, st, &grid);
@@ -227,7 +229,10 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
   b->d_ctl = reinterpret_cast<int*>(d + off_ctl);
   b->nctl = nctl;
   int grid = 0;
-  hipError_t e = b17::launch_batch(dtype, reinterpret_cast<const b17::BatchInst*>(d),
+  bool primes = false;
+  for (const b17::BatchInst& I : insts)
+    primes = primes || b17::plane_has_prime(I.pp) || b17::pencil_has_prime(I.pc);
+  hipError_t e = b17::launch_batch(dtype, primes, reinterpret_cast<const b17::BatchInst*>(d),
                                    reinterpret_cast<const int2*>(d + off_items), nitems, n_fwd,
                                    n_pen, b->d_ctl, nctl, threads, lds, b->spin_limit, st, &grid);
   if (e != hipSuccess) return b17_err(BRI17_ERR_HIP, hipGetErrorString(e));
@@ -243,6 +248,7 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
   b->last_threads = threads;
   b->last_lds = lds;
   b->last_dtype = dtype;
+  b->last_primes = primes;
   b->last_off_items = off_items;
   return BRI17_OK;
 }
@@ -273,7 +279,8 @@ int b17_batch_relaunch(bri17_batch* b, hipStream_t st) {
   if (!b || !b->d_buf || !b->last_nitems) return b17_err(BRI17_ERR_STATE, "no batch to relaunch");
   char* d = static_cast<char*>(b->d_buf);
   int grid = 0;
-  hipError_t e = b17::launch_batch(b->last_dtype, reinterpret_cast<const b17::BatchInst*>(d),
+  hipError_t e = b17::launch_batch(b->last_dtype, b->last_primes,
+                                   reinterpret_cast<const b17::BatchInst*>(d),
                                    reinterpret_cast<const int2*>(d + b->last_off_items),
                                    b->last_nitems, b->last_nfwd, b->last_npen, b->d_ctl, b->nctl,
                                    b->last_threads, b->last_lds, b->spin_limit, st, &grid);

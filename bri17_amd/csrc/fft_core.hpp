@@ -1,5 +1,5 @@
 // Device-side FFT engine for gfx950: Stockham autosort passes on sequences
-// staged in LDS, radix 2/3/4/5/7/8/9/16 butterflies held in VGPRs.
+// staged in LDS, radix 2/3/4/5/7/8/9/11/13/16 butterflies held in VGPRs.
 //
 // Replaces the FFTW c2c transforms of the reference apply harness
 // (tests/test_bri17.cpp:57,95, plans :119-127).  Conventions follow FFTW /
@@ -295,6 +295,78 @@ __device__ __forceinline__ void bfly16(cpx<T>* v) {
   for (int i = 0; i < 16; i++) v[i] = y[i];
 }
 
+// Odd prime radix R = 11, 13: the inputs pair as (k, R - k), k = 1..H,
+// H = (R - 1)/2, with t_k = v_k + v_{R-k} and d_k = v_k - v_{R-k}.  Output m
+// and R - m share A_m = v_0 + sum_k t_k cos(2 pi k m / R) and
+// B_m = sum_k d_k sin(2 pi k m / R): X_m = A_m -+ i B_m, X_{R-m} = A_m +- i B_m
+// (forward / inverse).  The index k m is reduced mod R at compile time.
+template <int R>
+struct PrimeRoots;
+template <>
+struct PrimeRoots<11> {  // cos, sin (2 pi j / 11), j = 1..5
+  static constexpr double c[6] = {1.0,
+                                  0.841253532831181168861811648919367718,
+                                  0.415415013001886425529274149229623204,
+                                  -0.142314838273285140443792668616369669,
+                                  -0.654860733945285064056925072466293553,
+                                  -0.959492973614497389890368057066327699};
+  static constexpr double s[6] = {0.0,
+                                  0.540640817455597582107635954318691695,
+                                  0.909631995354518371411715383079028460,
+                                  0.989821441880932732376092037776718787,
+                                  0.755749574354258283774035843972344420,
+                                  0.281732556841429697711417915346616899};
+};
+template <>
+struct PrimeRoots<13> {  // cos, sin (2 pi j / 13), j = 1..6
+  static constexpr double c[7] = {1.0,
+                                  0.885456025653209895900375522015098879,
+                                  0.568064746731155802511807559127516625,
+                                  0.120536680255323053349067687452543582,
+                                  -0.354604887042535625969637892600018474,
+                                  -0.748510748171101098634630599701351384,
+                                  -0.970941817426052027156982276293789227};
+  static constexpr double s[7] = {0.0,
+                                  0.464723172043768545656015335133104778,
+                                  0.822983865893656394579617423439381991,
+                                  0.992708874098053992800751649492520179,
+                                  0.935016242685414823439784599837830729,
+                                  0.663122658240795202376785492666766280,
+                                  0.239315664287557767148753726260211895};
+};
+
+template <int R, bool INV, typename T>
+__device__ __forceinline__ void bfly_prime(cpx<T>* v) {
+  constexpr int H = (R - 1) / 2;
+  cpx<T> t[H], d[H];
+#pragma unroll
+  for (int k = 1; k <= H; k++) {
+    t[k - 1] = v[k] + v[R - k];
+    d[k - 1] = v[k] - v[R - k];
+  }
+  cpx<T> y[R];
+  y[0] = v[0];
+#pragma unroll
+  for (int k = 0; k < H; k++) y[0] = y[0] + t[k];
+#pragma unroll
+  for (int m = 1; m <= H; m++) {
+    cpx<T> a = v[0], b = {T(0), T(0)};
+#pragma unroll
+    for (int k = 1; k <= H; k++) {
+      const int j = (k * m) % R;
+      const T c = T(PrimeRoots<R>::c[j <= H ? j : R - j]);
+      const T sn = j <= H ? T(PrimeRoots<R>::s[j]) : T(-PrimeRoots<R>::s[R - j]);
+      a = a + t[k - 1] * c;
+      b = b + d[k - 1] * sn;
+    }
+    const cpx<T> e = rot<INV>(b);
+    y[m] = a + e;
+    y[R - m] = a - e;
+  }
+#pragma unroll
+  for (int i = 0; i < R; i++) v[i] = y[i];
+}
+
 template <int R, bool INV, typename T>
 __device__ __forceinline__ void bfly(cpx<T>* v) {
   if constexpr (R == 2)
@@ -313,6 +385,8 @@ __device__ __forceinline__ void bfly(cpx<T>* v) {
     bfly9<INV>(v);
   else if constexpr (R == 16)
     bfly16<INV>(v);
+  else if constexpr (R == 11 || R == 13)
+    bfly_prime<R, INV>(v);
 }
 
 // ------------------------------------------------------------- LDS sequences
@@ -382,8 +456,10 @@ __device__ __forceinline__ void fft_pass(cpx<T>* buf, TWP tw,
 // thread of the workgroup (`wave`: of the wave, whose threads then hold
 // whole sequences; see seq_threads) must call this (barriers inside);
 // `active` marks threads that own a sequence.
-// `tw` is a pointer to const cpx<T> in any address space.
-template <typename T, bool INV, int MAXE, typename TWP>
+// `tw` is a pointer to const cpx<T> in any address space.  PR: the radix-11
+// and radix-13 passes are compiled in (their butterflies hold more values at
+// once; kernels without them keep the smaller register budget).
+template <typename T, bool INV, int MAXE, bool PR = false, typename TWP>
 __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
                                         const FftDesc& D, int base, int es,
                                         int lane, bool active, bool wave = false) {
@@ -413,6 +489,14 @@ __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
         break;
       case 16:
         fft_pass<T, 16, INV, MAXE / 16>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
+        break;
+      case 11:
+        if constexpr (PR)
+          fft_pass<T, 11, INV, MAXE / 11>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
+        break;
+      case 13:
+        if constexpr (PR)
+          fft_pass<T, 13, INV, MAXE / 13>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       default:
         break;

@@ -15,24 +15,26 @@
 namespace b17 {
 
 
-// Plain launches, one workgroup per plane block / pencil tile.
-template <typename T>
+// Plain launches, one workgroup per plane block / pencil tile.  PR: the
+// instantiation with the radix-11 and radix-13 passes, launched for plans
+// that use them.
+template <typename T, bool PR>
 __global__ void __launch_bounds__(1024)
     k_plane_fwd(const T* __restrict__ u, cpx<T>* __restrict__ ws, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  plane_fwd_body<T>((gptr<const T>)u, (gptr<cpx<T>>)ws, pp, blockIdx.x,
-                    reinterpret_cast<cpx<T>*>(smem));
+  plane_fwd_body<T, PR>((gptr<const T>)u, (gptr<cpx<T>>)ws, pp, blockIdx.x,
+                        reinterpret_cast<cpx<T>*>(smem));
 }
 
-template <typename T>
+template <typename T, bool PR>
 __global__ void __launch_bounds__(1024)
     k_plane_inv(const cpx<T>* __restrict__ ws, T* __restrict__ out, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
-  plane_inv_body<T>((gptr<const cpx<T>>)ws, (gptr<T>)out, pp, blockIdx.x,
-                    reinterpret_cast<cpx<T>*>(smem));
+  plane_inv_body<T, PR>((gptr<const cpx<T>>)ws, (gptr<T>)out, pp, blockIdx.x,
+                        reinterpret_cast<cpx<T>*>(smem));
 }
 
-template <typename T, int DIM, int MODE>
+template <typename T, int DIM, int MODE, bool PR>
 __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const PencilParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
   int t = blockIdx.x;
@@ -43,7 +45,7 @@ __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const 
     const int x = t % 8, idx = t / 8;
     t = x * per + (x < rem ? x : rem) + idx;
   }
-  pencil_body<T, DIM, MODE>((gptr<cpx<T>>)ws, pp, t, reinterpret_cast<cpx<T>*>(smem));
+  pencil_body<T, DIM, MODE, PR>((gptr<cpx<T>>)ws, pp, t, reinterpret_cast<cpx<T>*>(smem));
 }
 
 // ------------------------------------------------------------------ launchers
@@ -59,9 +61,10 @@ static hipError_t set_lds(K kernel, size_t lds) {
 template <typename T>
 hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp, int nblocks,
                             int nthreads, size_t lds, hipStream_t st) {
-  hipError_t e = set_lds(k_plane_fwd<T>, lds);
+  auto k = plane_has_prime(pp) ? k_plane_fwd<T, true> : k_plane_fwd<T, false>;
+  hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_plane_fwd<T>, dim3(nblocks), dim3(nthreads), lds, st, u,
+  hipLaunchKernelGGL(k, dim3(nblocks), dim3(nthreads), lds, st, u,
                      reinterpret_cast<cpx<T>*>(ws), pp);
   return hipGetLastError();
 }
@@ -69,9 +72,10 @@ hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp, int nbl
 template <typename T>
 hipError_t launch_plane_inv(const void* ws, T* out, const PlaneParams& pp, int nblocks,
                             int nthreads, size_t lds, hipStream_t st) {
-  hipError_t e = set_lds(k_plane_inv<T>, lds);
+  auto k = plane_has_prime(pp) ? k_plane_inv<T, true> : k_plane_inv<T, false>;
+  hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_plane_inv<T>, dim3(nblocks), dim3(nthreads), lds, st,
+  hipLaunchKernelGGL(k, dim3(nblocks), dim3(nthreads), lds, st,
                      reinterpret_cast<const cpx<T>*>(ws), out, pp);
   return hipGetLastError();
 }
@@ -79,9 +83,10 @@ hipError_t launch_plane_inv(const void* ws, T* out, const PlaneParams& pp, int n
 template <typename T, int DIM, int MODE>
 static hipError_t launch_pencil_t(void* ws, const PencilParams& pp, int nthreads, size_t lds,
                                   hipStream_t st) {
-  hipError_t e = set_lds(k_pencil<T, DIM, MODE>, lds);
+  auto k = pencil_has_prime(pp) ? k_pencil<T, DIM, MODE, true> : k_pencil<T, DIM, MODE, false>;
+  hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_pencil<T, DIM, MODE>), dim3(pp.ntiles), dim3(nthreads), lds, st,
+  hipLaunchKernelGGL(k, dim3(pp.ntiles), dim3(nthreads), lds, st,
                      reinterpret_cast<cpx<T>*>(ws), pp);
   return hipGetLastError();
 }

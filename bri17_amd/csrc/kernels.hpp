@@ -118,6 +118,20 @@ __host__ __device__ constexpr bool fused_split(int p) {
   return p == 1 || p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16;
 }
 
+// Whether a transform uses the radix-11 or radix-13 butterflies: those run in
+// separate kernel instantiations (template flag PR of the runtime-size bodies).
+__host__ __device__ inline bool fft_has_prime(const FftDesc& d) {
+  for (int i = 0; i < d.npass; i++)
+    if (d.pass[i].r == 11 || d.pass[i].r == 13) return true;
+  return false;
+}
+__host__ __device__ inline bool plane_has_prime(const PlaneParams& pp) {
+  return fft_has_prime(pp.row) || (pp.has_col && fft_has_prime(pp.col));
+}
+__host__ __device__ inline bool pencil_has_prime(const PencilParams& pc) {
+  return fft_has_prime(pc.ax0) || (pc.p > 1 && !fused_split(pc.p) && fft_has_prime(pc.pfft));
+}
+
 // Entries (complex T) of the twiddle tables a work item copies into LDS after
 // its data: plane items the row table (nl) and in 3D the middle-axis table
 // (n1); pencil tiles the axis-0 table (n0) and, for a split P without a fused

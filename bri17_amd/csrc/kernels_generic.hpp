@@ -13,6 +13,7 @@
 namespace b17 {
 
 constexpr int kMaxE = 16;  // max complex elements per thread per FFT pass
+
 // global loads a thread keeps in flight when staging (fewer for fp64: VGPRs)
 template <typename T>
 constexpr int kLoads = sizeof(T) == 4 ? 16 : 8;
@@ -96,7 +97,7 @@ __device__ __forceinline__ void copy_tables(const PlaneParams& pp, cpx<T>* twr, 
 
 // ------------------------------------------------------------------ plane fwd
 
-template <typename T>
+template <typename T, bool PR = false>
 __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
                                                gptr<cpx<T>> __restrict__ ws, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
     for (int r0 = 0; r0 < nseq; r0 += m.S) {
       const int s = r0 + m.sl;
       const bool act = m.sl < m.S && s < nseq;
-      lds_fft<T, false, kMaxE>(lds, twr, pp.row, act ? s * sst : 0, 1, m.lane, act, m.wave);
+      lds_fft<T, false, kMaxE, PR>(lds, twr, pp.row, act ? s * sst : 0, 1, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
@@ -207,7 +208,7 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
       const bool act = m.sl < m.S && s < ncol;
       const int g = act ? (int)fdiv((uint32_t)s, pp.nlhdiv) : 0;
       const int base = act ? g * pp.q * pp.rsl + (s - g * pp.nlh) : 0;
-      lds_fft<T, false, kMaxE>(lds, twc, pp.col, base, pp.rsl, m.lane, act, m.wave);
+      lds_fft<T, false, kMaxE, PR>(lds, twc, pp.col, base, pp.rsl, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
@@ -230,7 +231,7 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
 
 // ------------------------------------------------------------------ plane inv
 
-template <typename T>
+template <typename T, bool PR = false>
 __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ ws,
                                                gptr<T> __restrict__ out, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
@@ -281,7 +282,7 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
       const bool act = m.sl < m.S && s < ncol;
       const int g = act ? (int)fdiv((uint32_t)s, pp.nlhdiv) : 0;
       const int base = act ? g * pp.q * pp.rsl + (s - g * pp.nlh) : 0;
-      lds_fft<T, true, kMaxE>(lds, twc, pp.col, base, pp.rsl, m.lane, act, m.wave);
+      lds_fft<T, true, kMaxE, PR>(lds, twc, pp.col, base, pp.rsl, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
@@ -334,7 +335,7 @@ __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ w
     for (int r0 = 0; r0 < nseq; r0 += m.S) {
       const int s = r0 + m.sl;
       const bool act = m.sl < m.S && s < nseq;
-      lds_fft<T, true, kMaxE>(lds, twr, pp.row, act ? s * sst : 0, 1, m.lane, act, m.wave);
+      lds_fft<T, true, kMaxE, PR>(lds, twr, pp.row, act ? s * sst : 0, 1, m.lane, act, m.wave);
     }
     if (m.wave) __syncthreads();
   }
@@ -405,7 +406,7 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
 // P-point DFT for splits without a fused butterfly (P = 6, 9, 12, 27, ...):
 // a Stockham transform per (c, i0, w) column of the tile, P elements at
 // stride W, twiddles exp(-2 pi i j / P) = tw1[j q].
-template <typename T, bool INV>
+template <typename T, bool INV, bool PR>
 __device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const cpx<T>* tw1, const PencilParams& pp,
                                               int nouter, int tid, int nthr) {
   const int W = pp.w, P = pp.p;
@@ -416,11 +417,11 @@ __device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const cpx<T>* tw1, co
     const bool act = m.sl < m.S && s < nseq;
     const int outer = act ? (int)fdiv((uint32_t)s, pp.wdiv) : 0;
     const int base = act ? outer * P * W + (s - outer * W) : 0;
-    lds_fft<T, INV, kMaxE>(lds, tw1, pp.pfft, base, W, m.lane, act, m.wave);
+    lds_fft<T, INV, kMaxE, PR>(lds, tw1, pp.pfft, base, W, m.lane, act, m.wave);
   }
 }
 
-template <typename T, int DIM, int MODE>
+template <typename T, int DIM, int MODE, bool PR = false>
 __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const PencilParams& pp, int t,
                                             cpx<T>* lds) {
   constexpr int CIN = ModeComps<DIM, MODE>::cin, COUT = ModeComps<DIM, MODE>::cout;
@@ -458,7 +459,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     if (fused_split(P))
       pdft<T, false>(lds, P, W, pp.wdiv, CIN * N0, tid, nthr);
     else
-      pdft_stockham<T, false>(lds, tw1, pp, CIN * N0, tid, nthr);
+      pdft_stockham<T, false, PR>(lds, tw1, pp, CIN * N0, tid, nthr);
     __syncthreads();  // the axis-0 passes read columns other threads wrote
   }
 
@@ -468,7 +469,7 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     const bool act = m.sl < m.S && s < CIN * PW;
     const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
     const int base = act ? g * cst + (s - g * PW) : 0;
-    lds_fft<T, false, kMaxE>(lds, tw0, pp.ax0, base, PW, m.lane, act, m.wave);
+    lds_fft<T, false, kMaxE, PR>(lds, tw0, pp.ax0, base, PW, m.lane, act, m.wave);
   }
   if (m.wave) __syncthreads();
 
@@ -511,14 +512,14 @@ __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const 
     const bool act = m.sl < m.S && s < COUT * PW;
     const int g = act ? (int)fdiv((uint32_t)s, pp.pwdiv) : 0;
     const int base = act ? g * cst + (s - g * PW) : 0;
-    lds_fft<T, true, kMaxE>(lds, tw0, pp.ax0, base, PW, m.lane, act, m.wave);
+    lds_fft<T, true, kMaxE, PR>(lds, tw0, pp.ax0, base, PW, m.lane, act, m.wave);
   }
   if (m.wave) __syncthreads();
   if (P > 1) {
     if (fused_split(P))
       pdft<T, true>(lds, P, W, pp.wdiv, COUT * N0, tid, nthr);
     else
-      pdft_stockham<T, true>(lds, tw1, pp, COUT * N0, tid, nthr);
+      pdft_stockham<T, true, PR>(lds, tw1, pp, COUT * N0, tid, nthr);
     __syncthreads();  // the store reads entries other threads wrote
   }
 

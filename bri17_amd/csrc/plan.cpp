@@ -36,7 +36,7 @@ std::vector<int> default_radices(int n) {
   if (n % 4 == 0) r.push_back(4), n /= 4;
   if (n % 2 == 0) r.push_back(2), n /= 2;
   while (n % 9 == 0) r.push_back(9), n /= 9;
-  for (int p : {3, 5, 7})
+  for (int p : {3, 5, 7, 11, 13})
     while (n % p == 0) r.push_back(p), n /= p;
   if (n != 1) r.clear();
   return r;
@@ -55,7 +55,8 @@ bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
   }
   int prod = 1, rmax = 1;
   for (int r : radices) {
-    if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 9 || r == 16)) {
+    if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 9 || r == 11 ||
+          r == 13 || r == 16)) {
       if (why) *why = "unsupported radix " + std::to_string(r);
       return false;
     }
@@ -194,7 +195,7 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
     if (shape[d] < 1) return fail(1, "grid shape must be positive");
     if (!(L[d] > 0.0) || !std::isfinite(L[d])) return fail(1, "grid lengths must be positive and finite");
     if (default_radices(shape[d]).empty() && shape[d] != 1)
-      return fail(2, "grid side " + std::to_string(shape[d]) + " has a prime factor > 7");
+      return fail(2, "grid side " + std::to_string(shape[d]) + " has a prime factor > 13");
     pl->shape[d] = shape[d];
     pl->L[d] = L[d];
     nn *= shape[d];

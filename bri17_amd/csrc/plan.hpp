@@ -34,7 +34,7 @@ struct Tuning {
 };
 
 // Radix order (largest first) for a Stockham transform of length n, or empty
-// if n has a prime factor > 7.
+// if n has a prime factor > 13.
 std::vector<int> default_radices(int n);
 // want_wave: the stage may run wave-local passes (FftDesc::wave).
 bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,

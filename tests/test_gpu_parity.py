@@ -65,6 +65,9 @@ CASES = [
     ((243, 12), (1.0, 1.0)),
     ((512, 96), (1.0, 0.2)),
     ((1, 6, 5), (1.0, 1.0, 1.0)),
+    ((11, 13, 22), (1.0, 1.2, 0.8)),   # radix 11 and 13 (odd rows of 22 reals)
+    ((121, 26), (1.0, 1.0)),
+    ((13, 44, 39), (2.0, 1.0, 1.5)),
     ((64, 64, 64), (1.0, 1.0, 1.0)),    # fixed-size kernels (fp32 and fp64)
 ]
 
@@ -302,6 +305,30 @@ def test_middle_axis_splits_without_fused_butterfly(device, oracle, split, mode)
     fn = oracle.apply if mode == "apply" else oracle.solve
     ref = fn(shape, L, 1.4, 0.27, u.cpu().numpy())
     assert rel_err(got, ref) <= 1e-12
+
+
+@pytest.mark.parametrize("split", [11, 13])
+@pytest.mark.parametrize("mode", ["apply", "solve"])
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_prime_radix_splits(device, oracle, split, mode, dtype):
+    """A middle axis of 143 = 11 x 13 split by 11 or 13: the plane kernels
+    run the Q-point transform (13 or 11) and the pencil the split's P-point
+    Stockham transform with the radix-11 / radix-13 butterflies."""
+    import torch
+    from bri17_amd import StiffnessMatrix
+    shape, L = (10, 143, 12), (1.0, 1.3, 0.7)
+    op = StiffnessMatrix(shape, L, 1.4, 0.27, dtype, tuning=dict(plane_split=split))
+    assert op.describe()["plane"]["split"] == split
+    g = torch.Generator(device=device).manual_seed(split)
+    u = torch.rand((3,) + shape, generator=g, device=device, dtype=getattr(torch, dtype)) * 2 - 1
+    if mode == "solve":
+        u -= u.mean(dim=(1, 2, 3), keepdim=True)
+    got = (op.apply(u) if mode == "apply" else op.solve(u)).double().cpu().numpy()
+    fn = oracle.apply if mode == "apply" else oracle.solve
+    ref = fn(shape, L, 1.4, 0.27, u.double().cpu().numpy())
+    e = rel_err(got, ref)
+    print(shape, split, mode, dtype, "rel err %.2e" % e, flush=True)
+    assert e <= (1e-12 if dtype == "float64" else 1e-5)
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
