@@ -452,13 +452,72 @@ __device__ __forceinline__ void fft_pass(cpx<T>* buf, TWP tw,
   pass_sync(wave);
 }
 
+// Outputs a thread holds in a direct-DFT pass (the planner keeps n / tpt at
+// or below it for transforms with such a pass).
+constexpr int kMaxDft = 8;
+
+// Pass of a prime radix R above 16 (17, 19, ..., any prime): a direct DFT
+// per butterfly.  Output q of butterfly j (item e = q NR + j of the pass) is
+// sum_r w_R^(q r) t_r, t_r the twiddled input r; a thread forms its items
+// e = lane + i tpt one after the other (at most MAXE = kMaxDft, the
+// planner's bound for such passes), keeps them until the pass's
+// barrier and writes them in Stockham order.  The roots come from the
+// transform's table: w_R^s is entry s (N / R) tws.  O(R) operations per
+// entry: slower than the butterflies, for sides that have such a factor.
+template <typename T, bool INV, int MAXE, typename TWP>
+__device__ __forceinline__ void dft_pass(cpx<T>* buf, TWP tw, const PassDesc& P, int N, int tws,
+                                         int base, int es, int lane, int tpt, bool active,
+                                         bool wave) {
+  const int R = P.r, NR = P.nr, Ns = P.ns;
+  const int twm = P.twm * tws;     // inter-pass twiddle step
+  const int rstep = (N / R) * tws;  // table entry of w_R
+  cpx<T> o[MAXE];
+  int op[MAXE];
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < MAXE; i++) {
+      const int e = lane + i * tpt;
+      if (e < N) {
+        const int q = e / NR, j = e - q * NR;
+        const int qb = (int)fdiv((uint32_t)j, P.nsdiv), jm = j - qb * Ns;
+        const cpx<T>* src = buf + base + j * es;
+        cpx<T> acc = src[0];
+        int s = 0;           // q r mod R
+        int t = 0;           // jm r twm
+#pragma unroll 1
+        for (int r = 1; r < R; r++) {
+          s += q;
+          if (s >= R) s -= R;
+          t += jm * twm;
+          cpx<T> x = src[r * NR * es];
+          if (Ns > 1) x = INV ? cmulc(x, tw[t]) : cmul(x, tw[t]);
+          const cpx<T> w = tw[s * rstep];
+          acc = acc + (INV ? cmulc(x, w) : cmul(x, w));
+        }
+        o[i] = acc;
+        op[i] = base + (qb * Ns * R + jm + q * Ns) * es;
+      }
+    }
+  }
+  pass_sync(wave);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < MAXE; i++) {
+      const int e = lane + i * tpt;
+      if (e < N) buf[op[i]] = o[i];
+    }
+  }
+  pass_sync(wave);
+}
+
 // Full FFT of length D.n on the LDS sequence at `base` (stride es).  Every
 // thread of the workgroup (`wave`: of the wave, whose threads then hold
 // whole sequences; see seq_threads) must call this (barriers inside);
 // `active` marks threads that own a sequence.
-// `tw` is a pointer to const cpx<T> in any address space.  PR: the radix-11
-// and radix-13 passes are compiled in (their butterflies hold more values at
-// once; kernels without them keep the smaller register budget).
+// `tw` is a pointer to const cpx<T> in any address space.  PR: the passes of
+// prime radices from 11 up are compiled in (radix 11 and 13 butterflies,
+// direct DFTs above 16: they hold more values at once; kernels without them
+// keep the smaller register budget).
 template <typename T, bool INV, int MAXE, bool PR = false, typename TWP>
 __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
                                         const FftDesc& D, int base, int es,
@@ -499,6 +558,8 @@ __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
           fft_pass<T, 13, INV, MAXE / 13>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       default:
+        if constexpr (PR)
+          dft_pass<T, INV, kMaxDft>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
     }
   }

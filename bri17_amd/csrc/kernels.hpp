@@ -118,11 +118,12 @@ __host__ __device__ constexpr bool fused_split(int p) {
   return p == 1 || p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16;
 }
 
-// Whether a transform uses the radix-11 or radix-13 butterflies: those run in
-// separate kernel instantiations (template flag PR of the runtime-size bodies).
+// Whether a transform has a pass of a prime radix from 11 up (the radix-11
+// and radix-13 butterflies, direct DFTs above 16): those run in separate
+// kernel instantiations (template flag PR of the runtime-size bodies).
 __host__ __device__ inline bool fft_has_prime(const FftDesc& d) {
   for (int i = 0; i < d.npass; i++)
-    if (d.pass[i].r == 11 || d.pass[i].r == 13) return true;
+    if (d.pass[i].r == 11 || d.pass[i].r == 13 || d.pass[i].r > 16) return true;
   return false;
 }
 __host__ __device__ inline bool plane_has_prime(const PlaneParams& pp) {

@@ -38,8 +38,18 @@ std::vector<int> default_radices(int n) {
   while (n % 9 == 0) r.push_back(9), n /= 9;
   for (int p : {3, 5, 7, 11, 13})
     while (n % p == 0) r.push_back(p), n /= p;
-  if (n != 1) r.clear();
+  // larger prime factors: one direct-DFT pass each
+  for (int p = 17; (long long)p * p <= n; p += 2)
+    while (n % p == 0) r.push_back(p), n /= p;
+  if (n > 1) r.push_back(n);
   return r;
+}
+
+static bool is_prime(int n) {
+  if (n < 2) return false;
+  for (int p = 2; (long long)p * p <= n; p++)
+    if (n % p == 0) return false;
+  return true;
 }
 
 bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
@@ -56,7 +66,7 @@ bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
   int prod = 1, rmax = 1;
   for (int r : radices) {
     if (!(r == 2 || r == 3 || r == 4 || r == 5 || r == 7 || r == 8 || r == 9 || r == 11 ||
-          r == 13 || r == 16)) {
+          r == 13 || r == 16 || (r > 16 && is_prime(r)))) {
       if (why) *why = "unsupported radix " + std::to_string(r);
       return false;
     }
@@ -67,17 +77,22 @@ bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,
     if (why) *why = "radix order does not factor length " + std::to_string(n);
     return false;
   }
-  // threads per sequence: every pass must fit kMaxE elements per thread
+  // threads per sequence: every pass must fit kMaxE elements per thread (a
+  // direct-DFT pass of a prime above 16 holds its n / tpt outputs, at most 8)
   constexpr int kMaxE = 16;
-  int tpt = n / rmax;
+  int tpt = std::max(1, n / std::min(rmax, kMaxE));
   for (;; tpt++) {
     bool ok = true;
     for (int r : radices) {
       int nr = n / r;
       int per = (nr + tpt - 1) / tpt;
-      if (per * r > kMaxE) ok = false;
+      if (r > kMaxE ? (n + tpt - 1) / tpt > 8 : per * r > kMaxE) ok = false;  // 8: kMaxDft
     }
     if (ok) break;
+  }
+  if (tpt > 1024) {
+    if (why) *why = "length " + std::to_string(n) + " needs more than 1024 threads per sequence";
+    return false;
   }
   // Wave-local passes (seq_threads): the threads of a sequence fit one wave
   // when tpt is a power of two up to 64; tpt is rounded up to one where that
@@ -194,8 +209,6 @@ Status build_plan(int dim, const int* shape, const double* L, int dtype, const T
   for (int d = 0; d < dim; d++) {
     if (shape[d] < 1) return fail(1, "grid shape must be positive");
     if (!(L[d] > 0.0) || !std::isfinite(L[d])) return fail(1, "grid lengths must be positive and finite");
-    if (default_radices(shape[d]).empty() && shape[d] != 1)
-      return fail(2, "grid side " + std::to_string(shape[d]) + " has a prime factor > 13");
     pl->shape[d] = shape[d];
     pl->L[d] = L[d];
     nn *= shape[d];

@@ -34,7 +34,7 @@ struct Tuning {
 };
 
 // Radix order (largest first) for a Stockham transform of length n, or empty
-// if n has a prime factor > 13.
+// (any n >= 1: prime factors above 13 take one direct-DFT pass each).
 std::vector<int> default_radices(int n);
 // want_wave: the stage may run wave-local passes (FftDesc::wave).
 bool make_fft_desc(int n, int tws, const std::vector<int>& radices, FftDesc& d,

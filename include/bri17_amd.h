@@ -24,7 +24,7 @@ extern "C" {
 
 #define BRI17_OK 0
 #define BRI17_ERR_INVALID 1     /* bad argument (reference: ValueError in pybri17.cpp:13) */
-#define BRI17_ERR_UNSUPPORTED 2 /* e.g. a grid side with a prime factor > 13 */
+#define BRI17_ERR_UNSUPPORTED 2 /* e.g. a grid row that does not fit the LDS */
 #define BRI17_ERR_HIP 3         /* HIP runtime failure (no device, launch error) */
 #define BRI17_ERR_NOMEM 4       /* device allocation failed / budget exceeded */
 #define BRI17_ERR_STATE 5       /* call not valid in the object's current state */
@@ -83,9 +83,10 @@ typedef struct bri17_operator bri17_operator;
  * Hooke{mu, nu, grid}, bri17.hpp:193-194, plus the FFT plans of
  * StiffnessMatrixFactory, tests/test_bri17.cpp:110-128).  Builds the
  * device-resident twiddle and modal tables.  dtype: BRI17_F32 / BRI17_F64.
- * Grid sides must factor into 2, 3, 5, 7, 11, 13 (the reference's FFTW
- * plans take any side; a side with a larger prime factor is refused with
- * BRI17_ERR_UNSUPPORTED). */
+ * Any grid side, as with the reference's FFTW plans: prime factors up to 13
+ * run as butterflies, larger ones as direct-DFT passes (O(p) operations per
+ * entry).  Refused with BRI17_ERR_UNSUPPORTED: a transform that needs more
+ * than 1,024 threads per sequence or a row beyond the LDS of a workgroup. */
 int bri17_operator_create(int dim, const int* shape, const double* L, double mu,
                           double nu, int dtype, bri17_operator** out);
 int bri17_operator_destroy(bri17_operator* op);
@@ -96,7 +97,7 @@ int bri17_operator_destroy(bri17_operator* op);
 typedef struct bri17_tuning {
   int plane_split;    /* 3D: split P of the middle axis between the plane and
                          pencil kernels (a divisor of N1, at most 64, with
-                         prime factors <= 13) */
+                         any factorisation) */
   int pencil_width;   /* contiguous spectrum entries per pencil tile */
   int lds_budget_kb;  /* LDS per workgroup, KiB (<= 160) */
   int xcd_remap;      /* 0: off, 1: on, -1: automatic (off for the fixed-size

@@ -127,7 +127,7 @@ static void validation() {
         BRI17_ERR_INVALID);
   // operator creation: bad grids are refused before any device call
   bri17_operator* op = nullptr;
-  const int prime[3] = {17, 4, 4};  // a prime factor above 13
+  const int prime[3] = {4, 4, 65537};  // rows beyond a workgroup's LDS and threads
   CHECK(bri17_operator_create(4, shape, L, 1, 0.3, BRI17_F32, &op) == BRI17_ERR_INVALID);
   CHECK(bri17_operator_create(3, prime, L, 1, 0.3, BRI17_F32, &op) != BRI17_OK);
   CHECK(bri17_operator_create(3, shape, L, 1, 0.3, 7, &op) != BRI17_OK);

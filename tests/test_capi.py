@@ -46,8 +46,8 @@ def test_operator_create_rejects_bad_grids():
     from bri17_amd import _lib
     h = ctypes.c_void_p()
     L = (ctypes.c_double * 3)(1, 1, 1)
-    # a side with a prime factor above 13 is unsupported
-    for dim, shape, code in [(4, (4, 4, 4), 1), (3, (17, 4, 4), 2), (3, (0, 4, 4), 1)]:
+    # a row of 65,537 (prime) entries: beyond the LDS and the threads of a workgroup
+    for dim, shape, code in [(4, (4, 4, 4), 1), (3, (4, 4, 65537), 2), (3, (0, 4, 4), 1)]:
         sh = (ctypes.c_int * 3)(*shape)
         rc = _lib.lib().bri17_operator_create(dim, sh, L, 1.0, 0.3, 0, ctypes.byref(h))
         assert rc == code, (dim, shape, rc)

@@ -111,7 +111,7 @@ def test_batch_multi_plane_items_and_odd_rows(device, oracle, dtype):
     plane per item, so that no pair straddles two planes), an even last
     axis, a split-3 plan with odd rows and an odd Q, split-2 plans (one
     plane per item, rows strided in memory), and sides with the prime
-    radices 11 and 13.
+    radices 11, 13 and (direct-DFT passes) 17, 19, 23.
     Each instance equals its solo runtime-size result bitwise and the fp64
     oracle within tolerance."""
     import torch
@@ -121,7 +121,8 @@ def test_batch_multi_plane_items_and_odd_rows(device, oracle, dtype):
             ((8, 16, 45), 1.1, 0.1, "apply", 1), ((9, 24, 32), 1.7, 0.35, "solve", 1),
             ((6, 14, 81), 0.9, 0.25, "apply", 1), ((5, 81, 27), 1.2, 0.15, "apply", 3),
             ((6, 16, 27), 1.4, 0.3, "solve", 2), ((7, 24, 32), 0.6, 0.05, "apply", 2),
-            ((11, 26, 13), 1.0, 0.2, "apply", 1), ((13, 22, 11), 0.7, 0.3, "solve", 1)]
+            ((11, 26, 13), 1.0, 0.2, "apply", 1), ((13, 22, 11), 0.7, 0.3, "solve", 1),
+            ((17, 19, 23), 1.2, 0.25, "apply", 1)]
     g = torch.Generator(device=device).manual_seed(31)
     xs = []
     for shape, _, _, mode, _ in spec:

@@ -68,6 +68,9 @@ CASES = [
     ((11, 13, 22), (1.0, 1.2, 0.8)),   # radix 11 and 13 (odd rows of 22 reals)
     ((121, 26), (1.0, 1.0)),
     ((13, 44, 39), (2.0, 1.0, 1.5)),
+    ((17, 19, 23), (1.0, 1.1, 0.9)),   # prime factors above 13: direct-DFT passes
+    ((97, 34), (1.0, 1.0)),
+    ((31, 62, 8), (1.0, 2.0, 0.5)),
     ((64, 64, 64), (1.0, 1.0, 1.0)),    # fixed-size kernels (fp32 and fp64)
 ]
 
@@ -307,16 +310,17 @@ def test_middle_axis_splits_without_fused_butterfly(device, oracle, split, mode)
     assert rel_err(got, ref) <= 1e-12
 
 
-@pytest.mark.parametrize("split", [11, 13])
+@pytest.mark.parametrize("n1,split", [(143, 11), (143, 13), (221, 17), (221, 13)])
 @pytest.mark.parametrize("mode", ["apply", "solve"])
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
-def test_prime_radix_splits(device, oracle, split, mode, dtype):
-    """A middle axis of 143 = 11 x 13 split by 11 or 13: the plane kernels
-    run the Q-point transform (13 or 11) and the pencil the split's P-point
-    Stockham transform with the radix-11 / radix-13 butterflies."""
+def test_prime_radix_splits(device, oracle, n1, split, mode, dtype):
+    """A middle axis of 143 = 11 x 13 or 221 = 13 x 17 split by either
+    factor: the plane kernels run the Q-point transform and the pencil the
+    split's P-point Stockham transform, with the radix-11 / radix-13
+    butterflies and the direct-DFT pass of 17."""
     import torch
     from bri17_amd import StiffnessMatrix
-    shape, L = (10, 143, 12), (1.0, 1.3, 0.7)
+    shape, L = (10, n1, 12), (1.0, 1.3, 0.7)
     op = StiffnessMatrix(shape, L, 1.4, 0.27, dtype, tuning=dict(plane_split=split))
     assert op.describe()["plane"]["split"] == split
     g = torch.Generator(device=device).manual_seed(split)

@@ -197,8 +197,8 @@ def test_failures_are_isolated(device):
     xs = _inputs(device, spec, seed=14)
     with Server(max_batch=16) as srv:
         ref, _ = _serve(srv, spec, xs)
-        # unsupported grid (prime factor 17): rejected at submit, nothing queued
-        bad = torch.zeros((3, 17, 16, 16), device=device)
+        # unsupported grid (rows of 65,537 entries): rejected at submit, nothing queued
+        bad = torch.zeros((3, 4, 4, 65537), device=device)
         with pytest.raises(_lib.Bri17Error) as e:
             srv.submit(bad, torch.empty_like(bad))
         assert e.value.code == _lib.BRI17_ERR_UNSUPPORTED
