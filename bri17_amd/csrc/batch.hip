@@ -41,7 +41,7 @@ namespace b17 {
 // finished items a workgroup publishes with one release
 constexpr int kPend = 8;
 
-template <typename T, bool PR>
+template <typename T, int PR>
 __global__ void __launch_bounds__(1024)
     k_batch(const BatchInst* __restrict__ insts, const int2* __restrict__ items, int nitems,
             int n_fwd, int n_pen, int* __restrict__ ctl, int spin_limit) {
@@ -152,14 +152,14 @@ __global__ void k_batch_reset(int* __restrict__ ctl, int n) {
   if (i < n) ctl[i] = 0;
 }
 
-hipError_t launch_batch(int dtype, bool primes, const BatchInst* insts, const int2* items,
+hipError_t launch_batch(int dtype, int primes, const BatchInst* insts, const int2* items,
                         int nitems, int n_fwd, int n_pen, int* ctl, int nctl, int threads,
                         size_t lds, int spin_limit, hipStream_t st, int* grid_out) {
   hipLaunchKernelGGL(k_batch_reset, dim3((nctl + 255) / 256), dim3(256), 0, st, ctl, nctl);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  auto kf = primes ? k_batch<float, true> : k_batch<float, false>;
-  auto kd = primes ? k_batch<double, true> : k_batch<double, false>;
+  auto kf = primes == 2 ? k_batch<float, 2> : primes == 1 ? k_batch<float, 1> : k_batch<float, 0>;
+  auto kd = primes == 2 ? k_batch<double, 2> : primes == 1 ? k_batch<double, 1> : k_batch<double, 0>;
   const void* fn = dtype == 0 ? reinterpret_cast<const void*>(kf) : reinterpret_cast<const void*>(kd);
   if (lds > 65536) {
     e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -23,9 +23,9 @@ struct BatchInst {
 // forward items and pencil tiles of all instances.
 // ctl: [head, error, forward items done, pencil tiles done, done(2 per
 // instance)], nctl = 4 + 2 * ninst.
-// primes: some instance uses the radix-11 or radix-13 passes (the kernel
-// instantiation that holds them is launched).
-hipError_t launch_batch(int dtype, bool primes, const BatchInst* insts, const int2* items,
+// primes: the highest prime-radix tier of the instances (fft_prime_tier; the
+// kernel instantiation that holds those passes is launched).
+hipError_t launch_batch(int dtype, int primes, const BatchInst* insts, const int2* items,
                         int nitems, int n_fwd, int n_pen, int* ctl, int nctl, int threads,
                         size_t lds, int spin_limit, hipStream_t st, int* grid_out);
 

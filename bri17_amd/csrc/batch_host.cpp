@@ -25,7 +25,7 @@ struct bri17_batch {
   int spin_limit = 1 << 24;
   int last_nitems = 0, last_grid = 0, last_threads = 0, last_dtype = 0;
   int last_nfwd = 0, last_npen = 0, last_n = 0;
-  bool last_primes = false;  // the launch holds radix-11 / radix-13 transforms
+  int last_primes = 0;  // prime-radix tier of the launch (fft_prime_tier)
   hipStream_t last_stream = nullptr;
   size_t last_lds = 0, last_off_items = 0;
   int graphs = 0;  // live graphs captured from this context (they replay its tables)
@@ -229,9 +229,9 @@ int b17_batch_launch(bri17_batch* b, int dtype, const std::vector<b17::BatchInst
   b->d_ctl = reinterpret_cast<int*>(d + off_ctl);
   b->nctl = nctl;
   int grid = 0;
-  bool primes = false;
+  int primes = 0;
   for (const b17::BatchInst& I : insts)
-    primes = primes || b17::plane_has_prime(I.pp) || b17::pencil_has_prime(I.pc);
+    primes = std::max({primes, b17::plane_prime_tier(I.pp), b17::pencil_prime_tier(I.pc)});
   hipError_t e = b17::launch_batch(dtype, primes, reinterpret_cast<const b17::BatchInst*>(d),
                                    reinterpret_cast<const int2*>(d + off_items), nitems, n_fwd,
                                    n_pen, b->d_ctl, nctl, threads, lds, b->spin_limit, st, &grid);

@@ -515,10 +515,10 @@ __device__ __forceinline__ void dft_pass(cpx<T>* buf, TWP tw, const PassDesc& P,
 // whole sequences; see seq_threads) must call this (barriers inside);
 // `active` marks threads that own a sequence.
 // `tw` is a pointer to const cpx<T> in any address space.  PR: the passes of
-// prime radices from 11 up are compiled in (radix 11 and 13 butterflies,
-// direct DFTs above 16: they hold more values at once; kernels without them
-// keep the smaller register budget).
-template <typename T, bool INV, int MAXE, bool PR = false, typename TWP>
+// prime radices compiled in: 0 none, 1 the radix-11 and radix-13
+// butterflies, 2 also the direct DFTs of primes above 16 (they hold more
+// values at once; kernels without them keep the smaller register budget).
+template <typename T, bool INV, int MAXE, int PR = 0, typename TWP>
 __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
                                         const FftDesc& D, int base, int es,
                                         int lane, bool active, bool wave = false) {
@@ -550,15 +550,15 @@ __device__ __forceinline__ void lds_fft(cpx<T>* buf, TWP tw,
         fft_pass<T, 16, INV, MAXE / 16>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 11:
-        if constexpr (PR)
+        if constexpr (PR >= 1)
           fft_pass<T, 11, INV, MAXE / 11>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       case 13:
-        if constexpr (PR)
+        if constexpr (PR >= 1)
           fft_pass<T, 13, INV, MAXE / 13>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
       default:
-        if constexpr (PR)
+        if constexpr (PR >= 2)
           dft_pass<T, INV, kMaxDft>(buf, tw, P, D.n, D.tws, base, es, lane, D.tpt, active, wave);
         break;
     }

@@ -16,9 +16,8 @@ namespace b17 {
 
 
 // Plain launches, one workgroup per plane block / pencil tile.  PR: the
-// instantiation with the radix-11 and radix-13 passes, launched for plans
-// that use them.
-template <typename T, bool PR>
+// prime-radix tier compiled in (fft_prime_tier), chosen from the plan.
+template <typename T, int PR>
 __global__ void __launch_bounds__(1024)
     k_plane_fwd(const T* __restrict__ u, cpx<T>* __restrict__ ws, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -26,7 +25,7 @@ __global__ void __launch_bounds__(1024)
                         reinterpret_cast<cpx<T>*>(smem));
 }
 
-template <typename T, bool PR>
+template <typename T, int PR>
 __global__ void __launch_bounds__(1024)
     k_plane_inv(const cpx<T>* __restrict__ ws, T* __restrict__ out, const PlaneParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -34,7 +33,7 @@ __global__ void __launch_bounds__(1024)
                         reinterpret_cast<cpx<T>*>(smem));
 }
 
-template <typename T, int DIM, int MODE, bool PR>
+template <typename T, int DIM, int MODE, int PR>
 __global__ void __launch_bounds__(1024) k_pencil(cpx<T>* __restrict__ ws, const PencilParams pp) {
   extern __shared__ __align__(16) unsigned char smem[];
   int t = blockIdx.x;
@@ -61,7 +60,8 @@ static hipError_t set_lds(K kernel, size_t lds) {
 template <typename T>
 hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp, int nblocks,
                             int nthreads, size_t lds, hipStream_t st) {
-  auto k = plane_has_prime(pp) ? k_plane_fwd<T, true> : k_plane_fwd<T, false>;
+  const int pr = plane_prime_tier(pp);
+  auto k = pr == 2 ? k_plane_fwd<T, 2> : pr == 1 ? k_plane_fwd<T, 1> : k_plane_fwd<T, 0>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(nblocks), dim3(nthreads), lds, st, u,
@@ -72,7 +72,8 @@ hipError_t launch_plane_fwd(const T* u, void* ws, const PlaneParams& pp, int nbl
 template <typename T>
 hipError_t launch_plane_inv(const void* ws, T* out, const PlaneParams& pp, int nblocks,
                             int nthreads, size_t lds, hipStream_t st) {
-  auto k = plane_has_prime(pp) ? k_plane_inv<T, true> : k_plane_inv<T, false>;
+  const int pr = plane_prime_tier(pp);
+  auto k = pr == 2 ? k_plane_inv<T, 2> : pr == 1 ? k_plane_inv<T, 1> : k_plane_inv<T, 0>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(nblocks), dim3(nthreads), lds, st,
@@ -83,7 +84,10 @@ hipError_t launch_plane_inv(const void* ws, T* out, const PlaneParams& pp, int n
 template <typename T, int DIM, int MODE>
 static hipError_t launch_pencil_t(void* ws, const PencilParams& pp, int nthreads, size_t lds,
                                   hipStream_t st) {
-  auto k = pencil_has_prime(pp) ? k_pencil<T, DIM, MODE, true> : k_pencil<T, DIM, MODE, false>;
+  const int pr = pencil_prime_tier(pp);
+  auto k = pr == 2   ? k_pencil<T, DIM, MODE, 2>
+           : pr == 1 ? k_pencil<T, DIM, MODE, 1>
+                     : k_pencil<T, DIM, MODE, 0>;
   hipError_t e = set_lds(k, lds);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(k, dim3(pp.ntiles), dim3(nthreads), lds, st,

@@ -118,19 +118,28 @@ __host__ __device__ constexpr bool fused_split(int p) {
   return p == 1 || p == 2 || p == 3 || p == 4 || p == 5 || p == 7 || p == 8 || p == 16;
 }
 
-// Whether a transform has a pass of a prime radix from 11 up (the radix-11
-// and radix-13 butterflies, direct DFTs above 16): those run in separate
-// kernel instantiations (template flag PR of the runtime-size bodies).
-__host__ __device__ inline bool fft_has_prime(const FftDesc& d) {
-  for (int i = 0; i < d.npass; i++)
-    if (d.pass[i].r == 11 || d.pass[i].r == 13 || d.pass[i].r > 16) return true;
-  return false;
+// Prime-radix tier of a transform: 0 none, 1 a radix-11 or radix-13 pass, 2
+// a direct-DFT pass of a prime above 16.  Each tier runs in its own
+// instantiation of the runtime-size kernels (template parameter PR of their
+// bodies), so that those passes do not raise the register budget of the
+// others.
+__host__ __device__ inline int fft_prime_tier(const FftDesc& d) {
+  int t = 0;
+  for (int i = 0; i < d.npass; i++) {
+    const int r = d.pass[i].r;
+    if (r > 16) t = 2;
+    else if ((r == 11 || r == 13) && t < 1) t = 1;
+  }
+  return t;
 }
-__host__ __device__ inline bool plane_has_prime(const PlaneParams& pp) {
-  return fft_has_prime(pp.row) || (pp.has_col && fft_has_prime(pp.col));
+__host__ __device__ inline int plane_prime_tier(const PlaneParams& pp) {
+  const int a = fft_prime_tier(pp.row), b = pp.has_col ? fft_prime_tier(pp.col) : 0;
+  return a > b ? a : b;
 }
-__host__ __device__ inline bool pencil_has_prime(const PencilParams& pc) {
-  return fft_has_prime(pc.ax0) || (pc.p > 1 && !fused_split(pc.p) && fft_has_prime(pc.pfft));
+__host__ __device__ inline int pencil_prime_tier(const PencilParams& pc) {
+  const int a = fft_prime_tier(pc.ax0);
+  const int b = pc.p > 1 && !fused_split(pc.p) ? fft_prime_tier(pc.pfft) : 0;
+  return a > b ? a : b;
 }
 
 // Entries (complex T) of the twiddle tables a work item copies into LDS after

@@ -97,7 +97,7 @@ __device__ __forceinline__ void copy_tables(const PlaneParams& pp, cpx<T>* twr, 
 
 // ------------------------------------------------------------------ plane fwd
 
-template <typename T, bool PR = false>
+template <typename T, int PR = 0>
 __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
                                                gptr<cpx<T>> __restrict__ ws, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
@@ -231,7 +231,7 @@ __device__ __forceinline__ void plane_fwd_body(gptr<const T> __restrict__ u,
 
 // ------------------------------------------------------------------ plane inv
 
-template <typename T, bool PR = false>
+template <typename T, int PR = 0>
 __device__ __forceinline__ void plane_inv_body(gptr<const cpx<T>> __restrict__ ws,
                                                gptr<T> __restrict__ out, const PlaneParams& pp,
                                                int blk, cpx<T>* lds) {
@@ -406,7 +406,7 @@ __device__ __forceinline__ void pdft(cpx<T>* lds, int P, int W, FastDiv wdiv,
 // P-point DFT for splits without a fused butterfly (P = 6, 9, 12, 27, ...):
 // a Stockham transform per (c, i0, w) column of the tile, P elements at
 // stride W, twiddles exp(-2 pi i j / P) = tw1[j q].
-template <typename T, bool INV, bool PR>
+template <typename T, bool INV, int PR>
 __device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const cpx<T>* tw1, const PencilParams& pp,
                                               int nouter, int tid, int nthr) {
   const int W = pp.w, P = pp.p;
@@ -421,7 +421,7 @@ __device__ __forceinline__ void pdft_stockham(cpx<T>* lds, const cpx<T>* tw1, co
   }
 }
 
-template <typename T, int DIM, int MODE, bool PR = false>
+template <typename T, int DIM, int MODE, int PR = 0>
 __device__ __forceinline__ void pencil_body(gptr<cpx<T>> __restrict__ ws, const PencilParams& pp, int t,
                                             cpx<T>* lds) {
   constexpr int CIN = ModeComps<DIM, MODE>::cin, COUT = ModeComps<DIM, MODE>::cout;
